@@ -1,0 +1,841 @@
+// cf_api.hip -- host side of the fold engine: context, device memory, batch
+// packing, kernel launches, result readback, and the extern "C" boundary
+// declared in include/contrafold_amd.h.
+#include "cf_kernels.hip"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/contrafold_amd.h"
+
+#define CF_VERSION 1
+
+#include "cf_internal.h"
+
+namespace cfi {
+
+int fail(cf_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+int hipfail(cf_ctx* c, hipError_t e, const char* what) {
+    return fail(c, CF_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define CF_HIP(ctx, call, what)                                \
+    do {                                                       \
+        hipError_t _e = (call);                                \
+        if (_e != hipSuccess) return hipfail((ctx), _e, what); \
+    } while (0)
+
+inline uint8_t encode_base(char ch) {
+    switch (ch) {
+        case 'A': case 'a': return 0;
+        case 'C': case 'c': return 1;
+        case 'G': case 'g': return 2;
+        case 'U': case 'u': case 'T': case 't': return 3;
+        default: return 4;
+    }
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// carve consecutive sub-buffers out of one region
+struct Carver {
+    char* base;
+    size_t off = 0;
+    template <typename T>
+    T* take(int64_t count) {
+        off = align_up(off, 256);
+        T* p = (T*)(base + off);
+        off += (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+        return p;
+    }
+};
+template <typename F>
+size_t measure(F f) {
+    Carver c{nullptr};
+    f(c);
+    return align_up(c.off, 256);
+}
+
+int region_get(cf_ctx* c, Region& r, size_t n) {
+    if (r.p && r.n >= n) return CF_OK;
+    if (r.p) c->pool.put(r.p, r.n);
+    r = Region();
+    void* p = nullptr;
+    hipError_t e = c->pool.get(&p, n);
+    if (e != hipSuccess) return fail(c, CF_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
+    r.p = p;
+    r.n = DevPool::round(n);
+    return CF_OK;
+}
+void region_put(cf_ctx* c, Region& r) {
+    if (r.p) c->pool.put(r.p, r.n);
+    r = Region();
+}
+
+__global__ void cf_fill_f64(double* p, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int mea_lds(int Lmax) { return (int)align_up((size_t)(Lmax + 2) * 4 + 2 * (Lmax + 2) + 16, 16) + 2 * (Lmax + 2) * 4 + 16; }
+int vit_lds(int Lmax) {
+    return (int)align_up((size_t)(Lmax + 2) * 8 + 8 * 8 + 2 * (Lmax + 3) + 16, 16) + (4 * Lmax + 16) * 4 + 16;
+}
+
+// ---------------------------------------------------------------- stages
+int launch_inside(cf_batch* b) {
+    cf_ctx* c = b->ctx;
+    for (const auto& bk : b->buckets) {
+        const int nl = bk.end - bk.begin;
+        if (nl <= 0) continue;
+        CfLds plan = cf_lds_plan(bk.Lmax, true, 4);
+        const bool ring = plan.total <= c->lds_max;
+        if (!ring) plan = cf_lds_plan(bk.Lmax, false, 4);
+        if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the inside kernel's LDS plan");
+        if (ring)
+            hipLaunchKernelGGL(cf_inside_kernel<true>, dim3(nl), dim3(bk.threads), plan.total, b->stream, b->d_seqs,
+                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
+                               b->FCX, bk.Lmax);
+        else
+            hipLaunchKernelGGL(cf_inside_kernel<false>, dim3(nl), dim3(bk.threads), plan.total, b->stream, b->d_seqs,
+                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
+                               b->FCX, bk.Lmax);
+        CF_HIP(c, hipGetLastError(), "inside launch");
+    }
+    return CF_OK;
+}
+
+int launch_exterior(cf_batch* b, int mode) {
+    cf_ctx* c = b->ctx;
+    const int grid = (b->n + 63) / 64;
+    hipLaunchKernelGGL(cf_exterior_kernel, dim3(grid), dim3(128), 0, b->stream, b->d_seqs, b->d_order, b->n,
+                       b->d_codes, b->d_masks, c->dP32, b->FCX, b->F5, b->F5o, b->logz, mode);
+    CF_HIP(c, hipGetLastError(), "exterior launch");
+    return CF_OK;
+}
+
+int launch_outside(cf_batch* b) {
+    cf_ctx* c = b->ctx;
+    for (const auto& bk : b->buckets) {
+        const int nl = bk.end - bk.begin;
+        if (nl <= 0) continue;
+        CfLds plan = cf_lds_plan(bk.Lmax, true, 4);
+        const bool ring = plan.total <= c->lds_max;
+        if (!ring) plan = cf_lds_plan(bk.Lmax, false, 4);
+        if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
+        if (ring)
+            hipLaunchKernelGGL(cf_outside_kernel<true>, dim3(nl), dim3(bk.threads), plan.total, b->stream, b->d_seqs,
+                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
+                               b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST, bk.Lmax);
+        else
+            hipLaunchKernelGGL(cf_outside_kernel<false>, dim3(nl), dim3(bk.threads), plan.total, b->stream, b->d_seqs,
+                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
+                               b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST, bk.Lmax);
+        CF_HIP(c, hipGetLastError(), "outside launch");
+    }
+    return CF_OK;
+}
+
+int launch_mea(cf_batch* b) {
+    cf_ctx* c = b->ctx;
+    for (const auto& bk : b->buckets) {
+        const int nl = bk.end - bk.begin;
+        if (nl <= 0) continue;
+        const int lds = mea_lds(bk.Lmax);
+        if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the MEA kernel");
+        hipLaunchKernelGGL(cf_mea_kernel, dim3(nl * b->n_gamma), dim3(bk.threads), lds, b->stream, b->d_seqs,
+                           b->d_order + bk.begin, nl, b->d_codes, b->d_masks, b->POST, b->d_gin, b->SC, b->TB,
+                           b->total_T, b->mea_out, b->d_charoff, b->total_chars, bk.Lmax);
+        CF_HIP(c, hipGetLastError(), "mea launch");
+    }
+    return CF_OK;
+}
+
+int launch_viterbi(cf_batch* b, bool lse) {
+    cf_ctx* c = b->ctx;
+    for (const auto& bk : b->buckets) {
+        const int nl = bk.end - bk.begin;
+        if (nl <= 0) continue;
+        const int lds = vit_lds(bk.Lmax);
+        if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the Viterbi kernel");
+        if (lse)
+            hipLaunchKernelGGL(cf_viterbi_kernel<true>, dim3(nl), dim3(bk.threads), lds, b->stream, b->d_seqs,
+                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP64, b->VFC, b->VFM, b->VFM1,
+                               b->logz64, b->mfe_out, b->d_charoff, bk.Lmax);
+        else
+            hipLaunchKernelGGL(cf_viterbi_kernel<false>, dim3(nl), dim3(bk.threads), lds, b->stream, b->d_seqs,
+                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP64, b->VFC, b->VFM, b->VFM1,
+                               b->vscore, b->mfe_out, b->d_charoff, bk.Lmax);
+        CF_HIP(c, hipGetLastError(), "viterbi launch");
+    }
+    return CF_OK;
+}
+
+int launch_sample(cf_batch* b, uint64_t seed) {
+    cf_ctx* c = b->ctx;
+    hipLaunchKernelGGL(cf_sample_kernel, dim3(b->sgrid), dim3(64), 0, b->stream, b->d_seqs, b->d_order, b->n,
+                       b->n_samples, seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
+                       b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff);
+    CF_HIP(c, hipGetLastError(), "sample launch");
+    return CF_OK;
+}
+
+int launch_grad(cf_batch* b) {
+    cf_ctx* c = b->ctx;
+    hipLaunchKernelGGL(cf_fill_f64, dim3(4), dim3(256), 0, b->stream, b->grad, (int64_t)CF_NTAB, 0.0);
+    for (const auto& bk : b->buckets) {
+        const int nl = bk.end - bk.begin;
+        if (nl <= 0) continue;
+        hipLaunchKernelGGL(cf_grad_kernel, dim3(nl), dim3(bk.threads), 0, b->stream, b->d_seqs,
+                           b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
+                           b->F5, b->FCo, b->FMo, b->FM1o, b->F5o, (b->d_gin + kMaxGamma), b->grad);
+        CF_HIP(c, hipGetLastError(), "grad launch");
+    }
+    return CF_OK;
+}
+
+// Allocate whatever the requested stages need.
+int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
+    cf_ctx* c = b->ctx;
+    int rc;
+    if (flags & (CF_BPP | CF_MEA | CF_GRAD)) {
+        if (!b->r_out.p) {
+            size_t need = measure([&](Carver& k) {
+                for (int t = 0; t < 5; ++t) k.take<float>(b->total_T);
+            });
+            if ((rc = region_get(c, b->r_out, need))) return rc;
+            Carver k{(char*)b->r_out.p};
+            b->FCo = k.take<float>(b->total_T);
+            b->FMo = k.take<float>(b->total_T);
+            b->FM1o = k.take<float>(b->total_T);
+            b->FM2o = k.take<float>(b->total_T);
+            b->POST = k.take<float>(b->total_T);
+        }
+    }
+    if (flags & CF_MEA) {
+        if (n_gamma > b->mea_cap) {
+            region_put(c, b->r_mea);
+            auto lay = [&](Carver& k) {
+                k.take<float>((int64_t)n_gamma * b->total_T);
+                k.take<int16_t>((int64_t)n_gamma * b->total_T);
+                k.take<uint8_t>((int64_t)n_gamma * b->total_chars);
+            };
+            if ((rc = region_get(c, b->r_mea, measure(lay)))) return rc;
+            Carver k{(char*)b->r_mea.p};
+            b->SC = k.take<float>((int64_t)n_gamma * b->total_T);
+            b->TB = k.take<int16_t>((int64_t)n_gamma * b->total_T);
+            b->mea_out = k.take<uint8_t>((int64_t)n_gamma * b->total_chars);
+            b->mea_cap = n_gamma;
+        }
+    }
+    if ((flags & (CF_MFE | CF_LOGZ64)) && !b->r_vit.p) {
+        auto lay = [&](Carver& k) {
+            for (int t = 0; t < 3; ++t) k.take<double>(b->total_T);
+            k.take<double>(b->n);
+            k.take<double>(b->n);
+            k.take<uint8_t>(b->total_chars);
+        };
+        if ((rc = region_get(c, b->r_vit, measure(lay)))) return rc;
+        Carver k{(char*)b->r_vit.p};
+        b->VFC = k.take<double>(b->total_T);
+        b->VFM = k.take<double>(b->total_T);
+        b->VFM1 = k.take<double>(b->total_T);
+        b->vscore = k.take<double>(b->n);
+        b->logz64 = k.take<double>(b->n);
+        b->mfe_out = k.take<uint8_t>(b->total_chars);
+    }
+    if (flags & CF_SAMPLE) {
+        const int64_t tasks = (int64_t)b->n * n_samples;
+        const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>((tasks + 63) / 64, (int64_t)c->n_cu * 16));
+        const int cap = 2 * b->Lmax + 8;
+        if (n_samples > b->smp_cap || sgrid > b->sgrid || cap > b->stack_cap) {
+            region_put(c, b->r_smp);
+            const int ns_cap = std::max(n_samples, b->smp_cap);
+            const int g_cap = std::max(sgrid, b->sgrid);
+            auto lay = [&](Carver& k) {
+                k.take<uint8_t>((int64_t)ns_cap * b->total_chars);
+                k.take<uint32_t>((int64_t)g_cap * 64 * cap);
+            };
+            if ((rc = region_get(c, b->r_smp, measure(lay)))) return rc;
+            Carver k{(char*)b->r_smp.p};
+            b->samp_out = k.take<uint8_t>((int64_t)ns_cap * b->total_chars);
+            b->stack = k.take<uint32_t>((int64_t)g_cap * 64 * cap);
+            b->smp_cap = ns_cap;
+            b->stack_cap = cap;
+        }
+        b->sgrid = sgrid;
+    }
+    if ((flags & CF_GRAD) && !b->r_grad.p) {
+        auto lay = [&](Carver& k) {
+            k.take<double>(CF_NTAB);
+        };
+        if ((rc = region_get(c, b->r_grad, measure(lay)))) return rc;
+        Carver k{(char*)b->r_grad.p};
+        b->grad = k.take<double>(CF_NTAB);
+    }
+    return CF_OK;
+}
+
+// Enqueue the stages of one run on the context stream (no host sync, no
+// memset/memcpy: the sequence is capturable as a hipGraph).
+int enqueue_run(cf_batch* b, unsigned need, uint64_t seed, bool timing) {
+    cf_ctx* c = b->ctx;
+    int rc;
+    auto mark = [&](int k) {
+        if (timing) (void)hipEventRecord(b->ev[k], b->stream);
+    };
+    mark(0);
+    if (need & CF_LOGZ) {
+        if ((rc = launch_inside(b))) return rc;
+        mark(1);
+        if ((rc = launch_exterior(b, 1 | ((need & CF_BPP) ? 2 : 0)))) return rc;
+    } else {
+        mark(1);
+        // inside tables are resident; the outside still needs its exterior chain F5o
+        if ((need & CF_BPP) && (rc = launch_exterior(b, 2))) return rc;
+    }
+    mark(2);
+    if (need & CF_BPP) {
+        if ((rc = launch_outside(b))) return rc;
+    }
+    mark(3);
+    if (need & CF_MEA) {
+        if ((rc = launch_mea(b))) return rc;
+    }
+    mark(4);
+    if (need & CF_MFE) {
+        if ((rc = launch_viterbi(b, false))) return rc;
+    }
+    if (need & CF_LOGZ64) {
+        if ((rc = launch_viterbi(b, true))) return rc;
+    }
+    mark(5);
+    if (need & CF_SAMPLE) {
+        if ((rc = launch_sample(b, seed))) return rc;
+    }
+    mark(6);
+    if (need & CF_GRAD) {
+        if ((rc = launch_grad(b))) return rc;
+    }
+    mark(7);
+    return CF_OK;
+}
+
+
+size_t batch_staging_bytes(int n, int64_t C) {
+    return measure([&](Carver& k) {
+        k.take<CfSeq>(n);
+        k.take<int>(n);
+        k.take<uint8_t>(C);
+        k.take<uint8_t>(C);
+        k.take<uint64_t>(n);
+        k.take<int64_t>(n);
+        k.take<int64_t>(n);
+        k.take<float>(kMaxGamma + n);
+    });
+}
+
+int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
+                 const uint64_t* keys, hipStream_t stream, void** staging, size_t* staging_bytes, bool sync,
+                 cf_batch** out) {
+    if (!c || !out) return fail(c, CF_EINVAL, "null argument");
+    *out = nullptr;
+    if (n <= 0) return fail(c, CF_EINVAL, "empty batch");
+    if (!offsets || (!seqs && offsets[n] > 0)) return fail(c, CF_EINVAL, "null sequence buffer");
+    if (!c->dP32) return fail(c, CF_ENOPARAMS, "parameters not set (cf_set_params)");
+    CF_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    cf_batch* b = new cf_batch();
+    b->ctx = c;
+    b->stream = stream;
+    b->n = n;
+    b->L.resize(n);
+    b->charoff.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const int64_t l = offsets[i + 1] - offsets[i];
+        if (l < 0 || l > 32000) {
+            delete b;
+            return fail(c, CF_EINVAL, "sequence length out of range (0..32000)");
+        }
+        b->L[i] = (int)l;
+        b->charoff[i] = b->total_chars;
+        b->total_chars += l;
+        b->Lmax = std::max(b->Lmax, (int)l);
+    }
+    // launch order: longest first (largest work starts first), stable for ties
+    b->order.resize(n);
+    for (int i = 0; i < n; ++i) b->order[i] = i;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return b->L[x] > b->L[y]; });
+    {
+        int pos = 0;
+        const int nb = (int)c->edges.size();
+        for (int k = nb - 1; k >= 0; --k) {
+            const int lo = k ? c->edges[k - 1] : -1;
+            int end = pos;
+            while (end < n && b->L[b->order[end]] > lo) ++end;
+            if (end > pos) b->buckets.push_back({pos, end, c->threads[k], b->L[b->order[pos]]});
+            pos = end;
+        }
+    }
+    std::vector<CfSeq> sd(n);
+    int64_t C = 0, T = 0, X = 0;
+    for (int i = 0; i < n; ++i) {
+        sd[i].L = b->L[i];
+        sd[i].coff = (int32_t)C;
+        sd[i].toff = T;
+        sd[i].xoff = X;
+        C += b->L[i] + 2;
+        T += cf_tri_size(b->L[i]);
+        X += std::max<int64_t>(cf_x_size(b->L[i]), 1);
+    }
+    if (C > INT32_MAX) {
+        delete b;
+        return fail(c, CF_EINVAL, "batch too large (total residues)");
+    }
+    b->total_C = C;
+    b->total_T = T;
+    b->total_X = X;
+    const size_t in_bytes = batch_staging_bytes(n, C);
+    int rc = region_get(c, b->r_in, in_bytes);
+    if (rc) { delete b; return rc; }
+    auto lay_inside = [&](Carver& k) {
+        for (int t = 0; t < 4; ++t) k.take<float>(T);
+        k.take<float>(X);
+        k.take<float>(C);
+        k.take<float>(C);
+        k.take<float>(n);
+    };
+    rc = region_get(c, b->r_inside, measure(lay_inside));
+    if (rc) { region_put(c, b->r_in); delete b; return rc; }
+    {
+        Carver k{(char*)b->r_inside.p};
+        b->FC = k.take<float>(T);
+        b->FM = k.take<float>(T);
+        b->FM1 = k.take<float>(T);
+        b->FM2 = k.take<float>(T);
+        b->FCX = k.take<float>(X);
+        b->F5 = k.take<float>(C);
+        b->F5o = k.take<float>(C);
+        b->logz = k.take<float>(n);
+    }
+    if (*staging_bytes < in_bytes) {
+        if (*staging) (void)hipHostFree(*staging);
+        *staging = nullptr;
+        *staging_bytes = 0;
+        if (hipHostMalloc(staging, in_bytes) != hipSuccess) {
+            *staging = nullptr;
+            cf_batch_destroy(b);
+            return fail(c, CF_ENOMEM, "pinned staging allocation failed");
+        }
+        *staging_bytes = in_bytes;
+    }
+    Carver h{(char*)*staging};
+    CfSeq* h_seqs = h.take<CfSeq>(n);
+    int* h_order = h.take<int>(n);
+    uint8_t* h_codes = h.take<uint8_t>(C);
+    uint8_t* h_masks = h.take<uint8_t>(C);
+    uint64_t* h_keys = h.take<uint64_t>(n);
+    int64_t* h_charoff = h.take<int64_t>(n);
+    int64_t* h_soff = h.take<int64_t>(n);
+    std::memcpy(h_seqs, sd.data(), sizeof(CfSeq) * n);
+    std::memcpy(h_order, b->order.data(), sizeof(int) * n);
+    for (int i = 0; i < n; ++i) {
+        uint8_t* cs = h_codes + sd[i].coff;
+        uint8_t* ms = h_masks + sd[i].coff;
+        const int l = b->L[i];
+        cs[0] = 4;
+        cs[l + 1] = 4;
+        ms[0] = 0;
+        ms[l + 1] = 0;
+        const char* src = seqs ? seqs + offsets[i] : nullptr;
+        const uint8_t* msrc = masks ? masks + offsets[i] : nullptr;
+        for (int q = 0; q < l; ++q) {
+            cs[q + 1] = encode_base(src[q]);
+            ms[q + 1] = (msrc && msrc[q]) ? 1 : 0;
+        }
+        h_keys[i] = keys ? keys[i] : (uint64_t)i;
+        h_charoff[i] = b->charoff[i];
+        h_soff[i] = 0;
+    }
+    Carver d{(char*)b->r_in.p};
+    b->d_seqs = d.take<CfSeq>(n);
+    b->d_order = d.take<int>(n);
+    b->d_codes = d.take<uint8_t>(C);
+    uint8_t* dm = d.take<uint8_t>(C);
+    b->d_masks = masks ? dm : nullptr;
+    b->d_keys = d.take<uint64_t>(n);
+    b->d_charoff = d.take<int64_t>(n);
+    b->d_soff = d.take<int64_t>(n);
+    b->d_gin = d.take<float>(kMaxGamma + n);
+    for (auto& ev : b->ev) {
+        if (hipEventCreate(&ev) != hipSuccess) {
+            cf_batch_destroy(b);
+            return fail(c, CF_EDEVICE, "hipEventCreate failed");
+        }
+    }
+    // upload everything except the per-run tail (d_soff / d_gin are written by batch_enqueue)
+    const size_t head = (size_t)((char*)b->d_soff - (char*)b->r_in.p);
+    hipError_t e = hipMemcpyAsync(b->r_in.p, *staging, head, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && sync) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        cf_batch_destroy(b);
+        return hipfail(c, e, "upload batch");
+    }
+    *out = b;
+    return CF_OK;
+}
+
+int batch_enqueue(cf_batch* b, unsigned flags, const float* gammas, int n_gamma, int n_samples, uint64_t seed,
+                  const float* weights, void* staging, size_t staging_bytes, bool timing, unsigned* ran) {
+    cf_ctx* c = b->ctx;
+    unsigned need = flags & (CF_LOGZ | CF_BPP | CF_MEA | CF_MFE | CF_SAMPLE | CF_GRAD | CF_LOGZ64);
+    if (need & (CF_MEA | CF_GRAD)) need |= CF_BPP;
+    if (need & (CF_BPP | CF_SAMPLE)) need |= CF_LOGZ;
+    if (need & CF_MEA) {
+        if (!gammas || n_gamma <= 0 || n_gamma > kMaxGamma) return fail(c, CF_EINVAL, "MEA needs 1..64 gammas");
+        for (int g = 0; g < n_gamma; ++g)
+            if (!(gammas[g] > 0.0f) || !std::isfinite(gammas[g])) return fail(c, CF_EINVAL, "gamma must be > 0");
+    } else {
+        n_gamma = 0;
+    }
+    if (need & CF_SAMPLE) {
+        if (n_samples <= 0) return fail(c, CF_EINVAL, "n_samples must be > 0");
+    } else {
+        n_samples = 0;
+    }
+    int rc = ensure_buffers(b, need, n_gamma, n_samples);
+    if (rc) return rc;
+    b->n_gamma = n_gamma;
+    b->n_samples = n_samples;
+    if (b->params_version != c->params_version) b->done = 0;
+    unsigned run = need;
+    if (!(flags & CF_RECOMPUTE) && !(flags & CF_GRAPH)) {
+        if (b->done & CF_LOGZ) run &= ~CF_LOGZ;
+        if (b->done & CF_BPP) run &= ~CF_BPP;
+    }
+    // per-run inputs: sample offsets, gammas, gradient weights (pinned -> device, async)
+    const size_t tail = sizeof(int64_t) * (size_t)b->n + 256 + sizeof(float) * (kMaxGamma + (size_t)b->n);
+    if (staging_bytes < tail) return fail(c, CF_EINVAL, "staging buffer too small");
+    {
+        char* base = (char*)staging;
+        int64_t* so = (int64_t*)base;
+        for (int i = 0; i < b->n; ++i) so[i] = b->charoff[i] * (int64_t)n_samples;
+        const size_t gap = (size_t)((char*)b->d_gin - (char*)b->d_soff);
+        float* gw = (float*)(base + gap);
+        for (int g = 0; g < kMaxGamma; ++g) gw[g] = g < n_gamma ? gammas[g] : 1.0f;
+        for (int i = 0; i < b->n; ++i) gw[kMaxGamma + i] = weights ? weights[i] : 1.0f;
+        CF_HIP(c, hipMemcpyAsync(b->d_soff, staging, gap + sizeof(float) * (kMaxGamma + b->n), hipMemcpyHostToDevice,
+                                 b->stream),
+               "per-run inputs");
+    }
+    CF_HIP(c, hipEventRecord(b->ev[kNumKernels + 1], b->stream), "event");
+    if (flags & CF_GRAPH) {
+        const bool reuse = b->gexec && b->g_flags == run && b->g_ngamma == n_gamma && b->g_nsamp == n_samples &&
+                           b->g_seed == seed;
+        if (!reuse) {
+            if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+            if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+            CF_HIP(c, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+            rc = enqueue_run(b, run, seed, false);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return hipfail(c, e, "end capture");
+            b->graph = g;
+            CF_HIP(c, hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0), "graph instantiate");
+            b->g_flags = run;
+            b->g_ngamma = n_gamma;
+            b->g_nsamp = n_samples;
+            b->g_seed = seed;
+        }
+        CF_HIP(c, hipGraphLaunch(b->gexec, b->stream), "graph launch");
+    } else {
+        rc = enqueue_run(b, run, seed, timing);
+        if (rc) return rc;
+    }
+    CF_HIP(c, hipEventRecord(b->ev[kNumKernels + 2], b->stream), "event");
+    b->done |= need;
+    b->params_version = c->params_version;
+    if (ran) *ran = run;
+    return CF_OK;
+}
+
+}  // namespace cfi
+
+// =================================================================== C ABI
+using namespace cfi;
+
+extern "C" {
+
+int cf_version(void) { return CF_VERSION; }
+int cf_table_size(void) { return CF_NTAB; }
+int cf_kernel_count(void) { return cfi::kNumKernels; }
+const char* cf_kernel_name(int k) { return (k >= 0 && k < cfi::kNumKernels) ? cfi::kKernelNames[k] : ""; }
+
+int cf_create(int device, cf_ctx** out) {
+    if (!out) return CF_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CF_EDEVICE;
+    if (device < 0 || device >= ndev) return CF_EINVAL;
+    cf_ctx* c = new cf_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return CF_EDEVICE; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CF_EDEVICE; }
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+        c->lds_max = v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0) c->n_cu = v;
+    const void* dyn[] = {(const void*)cf_inside_kernel<true>, (const void*)cf_inside_kernel<false>,
+                         (const void*)cf_outside_kernel<true>, (const void*)cf_outside_kernel<false>,
+                         (const void*)cf_mea_kernel, (const void*)cf_viterbi_kernel<false>,
+                         (const void*)cf_viterbi_kernel<true>};
+    for (const void* f : dyn) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
+    (void)hipGetLastError();
+    *out = c;
+    return CF_OK;
+}
+
+void cf_destroy(cf_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->dP32) (void)hipFree(c->dP32);
+    if (c->dP64) (void)hipFree(c->dP64);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pool.trim();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* cf_last_error(const cf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int cf_set_params(cf_ctx* c, const double* tables) {
+    if (!c || !tables) return fail(c, CF_EINVAL, "null argument");
+    CF_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    CfParams<float> h32;
+    CfParams<double> h64;
+    for (int k = 0; k < CF_NTAB; ++k) {
+        if (!std::isfinite(tables[k])) return fail(c, CF_EINVAL, "non-finite parameter value");
+        h64.t[k] = tables[k];
+        h32.t[k] = (float)tables[k];  // cpu._params_f32: FP64 tables cast to float32
+    }
+    for (int a = 0; a < 5; ++a)
+        for (int bb = 0; bb < 5; ++bb)
+            for (int cc = 0; cc < 5; ++cc)
+                for (int d = 0; d < 5; ++d) {
+                    const int q = cf_i4(a, bb, cc, d);
+                    // the reference's ScoreJunctionB: hc[a][b] + tm[a][b][c][d], one add in each precision
+                    volatile float hf = h32.t[CF_O_HC + cf_i2(a, bb)];
+                    volatile float tf = h32.t[CF_O_TM + q];
+                    h32.JB[q] = hf + tf;
+                    h64.JB[q] = h64.t[CF_O_HC + cf_i2(a, bb)] + h64.t[CF_O_TM + q];
+                }
+    if (!c->dP32) CF_HIP(c, hipMalloc((void**)&c->dP32, sizeof(CfParams<float>)), "hipMalloc params");
+    if (!c->dP64) CF_HIP(c, hipMalloc((void**)&c->dP64, sizeof(CfParams<double>)), "hipMalloc params");
+    CF_HIP(c, hipStreamSynchronize(c->stream), "sync");
+    CF_HIP(c, hipMemcpy(c->dP32, &h32, sizeof(h32), hipMemcpyHostToDevice), "upload params");
+    CF_HIP(c, hipMemcpy(c->dP64, &h64, sizeof(h64), hipMemcpyHostToDevice), "upload params");
+    ++c->params_version;
+    return CF_OK;
+}
+
+int cf_set_tuning(cf_ctx* c, const int* edges, const int* threads, int nb) {
+    if (!c || !edges || !threads || nb <= 0) return fail(c, CF_EINVAL, "bad tuning arguments");
+    std::vector<int> e(edges, edges + nb), t(threads, threads + nb);
+    for (int k = 0; k < nb; ++k) {
+        if (t[k] < 64 || t[k] > 256 || t[k] % 64) return fail(c, CF_EINVAL, "threads must be 64, 128, 192 or 256");
+        if (k && e[k] <= e[k - 1]) return fail(c, CF_EINVAL, "bucket edges must increase");
+    }
+    e.back() = 1 << 30;
+    c->edges = e;
+    c->threads = t;
+    return CF_OK;
+}
+
+int64_t cf_fold_bytes(int L, unsigned flags, int n_gamma, int n_samples) {
+    if (L < 0) return -1;
+    const int64_t T = cf_tri_size(L), X = cf_x_size(L);
+    int64_t bytes = (L + 2) * 2 + sizeof(CfSeq) + 64;  // codes, masks, descriptor
+    bytes += (4 * T + X + 2 * (L + 2) + 1) * 4;        // inside tables, FCX, F5, F5o, logZ
+    if (flags & (CF_BPP | CF_MEA | CF_GRAD)) bytes += 5 * T * 4;
+    if (flags & CF_MEA) bytes += (int64_t)std::max(1, n_gamma) * (T * 6 + L);
+    if (flags & CF_MFE) bytes += 3 * T * 8 + 8 + L;
+    if (flags & CF_SAMPLE) bytes += (int64_t)std::max(1, n_samples) * L;
+    return bytes;
+}
+
+int cf_batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
+                    const uint64_t* keys, cf_batch** out) {
+    if (!c) return CF_EINVAL;
+    return cfi::batch_create(c, n, seqs, offsets, masks, keys, c->stream, &c->pinned, &c->pinned_bytes, true, out);
+}
+
+int cf_batch_run(cf_batch* b, unsigned flags, const float* gammas, int n_gamma, int n_samples, uint64_t seed,
+                 const float* grad_weights) {
+    if (!b) return CF_EINVAL;
+    cf_ctx* c = b->ctx;
+    CF_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    const size_t need_stage = sizeof(float) * (cfi::kMaxGamma + (size_t)b->n) + sizeof(int64_t) * (size_t)b->n + 4096;
+    if (c->pinned_bytes < need_stage) {
+        CF_HIP(c, hipStreamSynchronize(b->stream), "sync");
+        if (c->pinned) (void)hipHostFree(c->pinned);
+        c->pinned = nullptr;
+        c->pinned_bytes = 0;
+        CF_HIP(c, hipHostMalloc(&c->pinned, need_stage), "pinned staging");
+        c->pinned_bytes = need_stage;
+    }
+    unsigned ran = 0;
+    int rc = cfi::batch_enqueue(b, flags, gammas, n_gamma, n_samples, seed, grad_weights, c->pinned,
+                                c->pinned_bytes, (flags & CF_TIMING) != 0, &ran);
+    if (rc) {
+        (void)hipStreamSynchronize(b->stream);
+        return rc;
+    }
+    CF_HIP(c, hipStreamSynchronize(b->stream), "fold run");
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b->ev[cfi::kNumKernels + 1], b->ev[cfi::kNumKernels + 2]) == hipSuccess)
+        b->last_ms = ms;
+    if ((flags & CF_TIMING) && !(flags & CF_GRAPH)) {
+        for (int k = 0; k < cfi::kNumKernels; ++k) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, b->ev[k], b->ev[k + 1]) == hipSuccess) b->kms[k] = t;
+        }
+    }
+    return CF_OK;
+}
+
+int cf_batch_logz(const cf_batch* b, float* out) {
+    if (!b || !out) return CF_EINVAL;
+    cf_ctx* c = b->ctx;
+    if (!(b->done & CF_LOGZ)) return fail(c, CF_ESTATE, "logZ not computed");
+    CF_HIP(c, hipMemcpy(out, b->logz, sizeof(float) * b->n, hipMemcpyDeviceToHost), "read logz");
+    return CF_OK;
+}
+
+int cf_batch_bpp(const cf_batch* b, int index, float* out) {
+    if (!b || !out || index < 0 || index >= b->n) return CF_EINVAL;
+    cf_ctx* c = b->ctx;
+    if (!(b->done & CF_BPP)) return fail(c, CF_ESTATE, "bpp not computed");
+    const int L = b->L[index];
+    int64_t toff = 0;
+    for (int i = 0; i < index; ++i) toff += cf_tri_size(b->L[i]);
+    std::vector<float> tri((size_t)cf_tri_size(L));
+    CF_HIP(c, hipMemcpy(tri.data(), b->POST + toff, sizeof(float) * tri.size(), hipMemcpyDeviceToHost), "read bpp");
+    std::fill(out, out + (size_t)L * L, 0.0f);
+    for (int a = 1; a <= L; ++a)
+        for (int cc = a + 1; cc <= L; ++cc) out[(size_t)(a - 1) * L + (cc - 1)] = tri[cf_tri(L, a, cc - 1)];
+    return CF_OK;
+}
+
+int cf_batch_mea(const cf_batch* b, int g, char* out) {
+    if (!b || !out) return CF_EINVAL;
+    cf_ctx* c = b->ctx;
+    if (!(b->done & CF_MEA) || g < 0 || g >= b->n_gamma) return fail(c, CF_ESTATE, "MEA not computed for this gamma");
+    if (b->total_chars)
+        CF_HIP(c, hipMemcpy(out, b->mea_out + (int64_t)g * b->total_chars, b->total_chars, hipMemcpyDeviceToHost),
+               "read mea");
+    return CF_OK;
+}
+
+int cf_batch_mfe(const cf_batch* b, char* out, double* scores) {
+    if (!b || !out) return CF_EINVAL;
+    cf_ctx* c = b->ctx;
+    if (!(b->done & CF_MFE)) return fail(c, CF_ESTATE, "MFE not computed");
+    if (b->total_chars) CF_HIP(c, hipMemcpy(out, b->mfe_out, b->total_chars, hipMemcpyDeviceToHost), "read mfe");
+    if (scores) CF_HIP(c, hipMemcpy(scores, b->vscore, sizeof(double) * b->n, hipMemcpyDeviceToHost), "read scores");
+    return CF_OK;
+}
+
+int cf_batch_samples(const cf_batch* b, char* out) {
+    if (!b || !out) return CF_EINVAL;
+    cf_ctx* c = b->ctx;
+    if (!(b->done & CF_SAMPLE)) return fail(c, CF_ESTATE, "samples not computed");
+    const int64_t bytes = (int64_t)b->n_samples * b->total_chars;
+    if (bytes) CF_HIP(c, hipMemcpy(out, b->samp_out, bytes, hipMemcpyDeviceToHost), "read samples");
+    return CF_OK;
+}
+
+int cf_batch_table(const cf_batch* b, int index, int which, float* out) {
+    if (!b || !out || index < 0 || index >= b->n || which < 0 || which > 10) return CF_EINVAL;
+    cf_ctx* c = b->ctx;
+    const bool outside = which >= 4 && which != 9;
+    if (!(b->done & CF_LOGZ) || (outside && !(b->done & CF_BPP))) return fail(c, CF_ESTATE, "table not computed");
+    const int L = b->L[index];
+    int64_t toff = 0, coff = 0;
+    for (int i = 0; i < index; ++i) {
+        toff += cf_tri_size(b->L[i]);
+        coff += b->L[i] + 2;
+    }
+    const float* src = nullptr;
+    int64_t cnt = cf_tri_size(L);
+    switch (which) {
+        case 0: src = b->FC + toff; break;
+        case 1: src = b->FM + toff; break;
+        case 2: src = b->FM1 + toff; break;
+        case 3: src = b->FM2 + toff; break;
+        case 4: src = b->FCo + toff; break;
+        case 5: src = b->FMo + toff; break;
+        case 6: src = b->FM1o + toff; break;
+        case 7: src = b->FM2o + toff; break;
+        case 8: src = b->POST + toff; break;
+        case 9: src = b->F5 + coff; cnt = L + 2; break;
+        default: src = b->F5o + coff; cnt = L + 2; break;
+    }
+    CF_HIP(c, hipMemcpy(out, src, sizeof(float) * cnt, hipMemcpyDeviceToHost), "read table");
+    return CF_OK;
+}
+
+int cf_batch_logz64(const cf_batch* b, double* out) {
+    if (!b || !out) return CF_EINVAL;
+    cf_ctx* c = b->ctx;
+    if (!(b->done & CF_LOGZ64)) return fail(c, CF_ESTATE, "FP64 logZ not computed");
+    CF_HIP(c, hipMemcpy(out, b->logz64, sizeof(double) * b->n, hipMemcpyDeviceToHost), "read logz64");
+    return CF_OK;
+}
+
+int cf_batch_grad(const cf_batch* b, double* out) {
+    if (!b || !out) return CF_EINVAL;
+    cf_ctx* c = b->ctx;
+    if (!(b->done & CF_GRAD)) return fail(c, CF_ESTATE, "gradient not computed");
+    CF_HIP(c, hipMemcpy(out, b->grad, sizeof(double) * CF_NTAB, hipMemcpyDeviceToHost), "read grad");
+    return CF_OK;
+}
+
+float cf_batch_last_ms(const cf_batch* b) { return b ? b->last_ms : 0.0f; }
+
+int cf_batch_kernel_ms(const cf_batch* b, float* out) {
+    if (!b || !out) return CF_EINVAL;
+    std::copy(b->kms, b->kms + cfi::kNumKernels, out);
+    return CF_OK;
+}
+
+int64_t cf_batch_device_bytes(const cf_batch* b) { return b ? b->device_bytes() : 0; }
+
+void cf_batch_destroy(cf_batch* b) {
+    if (!b) return;
+    cf_ctx* c = b->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto& ev : b->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    region_put(c, b->r_in);
+    region_put(c, b->r_inside);
+    region_put(c, b->r_out);
+    region_put(c, b->r_mea);
+    region_put(c, b->r_vit);
+    region_put(c, b->r_smp);
+    region_put(c, b->r_grad);
+    delete b;
+}
+
+}  // extern "C"

@@ -1,0 +1,159 @@
+// cf_device.h -- device-side building blocks of the gfx950 fold engine.
+//
+// Shared by every kernel: the FP32 log-space operators (CONTRAfold's
+// Fast_LogPlusEquals / Fast_Exp, restated from gpu_contrafold/cpu.py:527-580),
+// the triangular diagonal-major table layout, and the parameter block.
+//
+// All HIP sources are compiled with -ffp-contract=off: every a*b+c below is a
+// rounded multiply followed by a rounded add, exactly like the reference's
+// float32 path, which is what makes the FP32 tables bit-identical to it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CF_NEG (-1e30f)
+#define CF_HALF (-5e29f)
+#define CF_NEG64 (-1e30)
+#define CF_HALF64 (-5e29)
+#define CF_CMAX 30
+#define CF_RING 33  // FC diagonals kept in LDS: the window reads s-2 .. s-32
+
+// Flat parameter-table layout shared with the host (2577 values; the order
+// of the dict returned by cpu.load, cpu.py:132-135).
+#define CF_O_BP 0
+#define CF_O_STACK 25
+#define CF_O_TM 650
+#define CF_O_HC 1275
+#define CF_O_DL 1300
+#define CF_O_DR 1425
+#define CF_O_HP 1550
+#define CF_O_CS 1581
+#define CF_O_I11 2542
+#define CF_O_B01 2567
+#define CF_O_MB 2572
+#define CF_O_MU 2573
+#define CF_O_MP 2574
+#define CF_O_EU 2575
+#define CF_O_EP 2576
+#define CF_NTAB 2577
+
+// Device parameter block.  `t` holds the 2577 tables in the flat order; JB is
+// the derived junction table hc[a][b] + tm[a][b][c][d] (one rounded add, the
+// same add the reference performs for every ScoreJunctionB).
+template <typename T>
+struct CfParams {
+    T t[CF_NTAB];
+    T JB[625];
+};
+
+__host__ __device__ constexpr int cf_i2(int a, int b) { return a * 5 + b; }
+__host__ __device__ constexpr int cf_i3(int a, int b, int c) { return (a * 5 + b) * 5 + c; }
+__host__ __device__ constexpr int cf_i4(int a, int b, int c, int d) { return ((a * 5 + b) * 5 + c) * 5 + d; }
+
+// canonical pairs AU UA CG GC GU UG: bit (5*a+b) of a 25-bit mask
+#define CF_CANON_MASK ((1u << (0 * 5 + 3)) | (1u << (3 * 5 + 0)) | (1u << (1 * 5 + 2)) | (1u << (2 * 5 + 1)) | \
+                       (1u << (2 * 5 + 3)) | (1u << (3 * 5 + 2)))
+__host__ __device__ inline bool cf_canon(int a, int b) { return (CF_CANON_MASK >> (a * 5 + b)) & 1u; }
+
+// ---------------------------------------------------------------- layout
+// Triangular, diagonal-major: cell (i, j), 0 <= i <= j <= L, lives at
+// dbase(j - i) + i with dbase(d) = d*(L+1) - d*(d-1)/2.  Lanes that walk a
+// diagonal touch consecutive addresses, and so do the FM1-row / FM-column
+// reads of the multiloop split at a fixed split offset.
+__host__ __device__ inline int64_t cf_dbase(int L, int d) {
+    return (int64_t)d * (L + 1) - (int64_t)d * (d - 1) / 2;
+}
+__host__ __device__ inline int64_t cf_tri(int L, int i, int j) { return cf_dbase(L, j - i) + i; }
+__host__ __device__ inline int64_t cf_tri_size(int L) { return (int64_t)(L + 1) * (L + 2) / 2; }
+// Exterior (column-major) copy of FC used by the F5 / F5o chains: entry
+// (k, j), 0 <= k < j <= L, holds FC[k+1][j-1].
+__host__ __device__ inline int64_t cf_xbase(int j) { return (int64_t)j * (j - 1) / 2; }
+__host__ __device__ inline int64_t cf_x_size(int L) { return (int64_t)L * (L + 1) / 2; }
+
+// Per-sequence descriptor (host builds it; one per sequence of a batch).
+struct CfSeq {
+    int32_t L;
+    int32_t coff;   // offset of s[0] in codes / masks (L+2 entries), also F5/F5o/up arrays
+    int64_t toff;   // offset of cell (0,0) in every triangular table
+    int64_t xoff;   // offset of the exterior FC copy
+};
+
+// ---------------------------------------------------------------- LSE
+// CONTRAfold Fast_LogExpPlusOne segments for RealT=float (cpu.py:538-556):
+// r(d) = ((c0*d + c1)*d + c2)*d + c3 on 8 consecutive d-intervals.
+struct CfLseTab {
+    float4 c[8];
+};
+__constant__ const float cf_lse_coef[8][4] = {
+    {-0.0065591595f, 0.1276442762f, 0.4996554598f, 0.6931542306f},
+    {-0.0155157557f, 0.1446775699f, 0.4882939746f, 0.6958092989f},
+    {-0.0128909247f, 0.1301028251f, 0.5150398748f, 0.6795585882f},
+    {-0.0072142647f, 0.0877540853f, 0.6208708362f, 0.5909675829f},
+    {-0.0031455354f, 0.0467229449f, 0.7592532310f, 0.4348794399f},
+    {-0.0010110698f, 0.0185943421f, 0.8831730747f, 0.2523695427f},
+    {-0.0001962780f, 0.0046084408f, 0.9634431978f, 0.0983148903f},
+    {-0.0000113994f, 0.0003734731f, 0.9959107193f, 0.0149855051f},
+};
+
+// log(exp(a)+exp(b)), float32, branch-free.  Same result bits as the
+// reference's branchy lse32 (cpu.py:527-557): the interval of d is found by a
+// three-level compare tree over the 7 breakpoints, the cubic's coefficients
+// come from a table in LDS, and the two early exits become one select.
+__device__ __forceinline__ float cf_lse(float a, float b, const float4* __restrict__ C) {
+    const float hi = fmaxf(a, b);
+    const float lo = fminf(a, b);
+    const float d = hi - lo;
+    const bool b2 = d >= 3.3792499610f;
+    const bool b1 = d >= (b2 ? 5.7890710412f : 1.6320158198f);
+    const float t0 = b2 ? (b1 ? 7.8162726752f : 4.4261691294f) : (b1 ? 2.4912588184f : 0.6615367791f);
+    const bool b0 = d >= t0;
+    const int seg = (b2 ? 4 : 0) | (b1 ? 2 : 0) | (b0 ? 1 : 0);
+    const float4 c = C[seg];
+    float r = c.x * d;
+    r = r + c.y;
+    r = r * d;
+    r = r + c.z;
+    r = r * d;
+    r = r + c.w;
+    const float res = lo + r;
+    return (lo < CF_HALF || d >= 11.8624794162f) ? hi : res;
+}
+
+// CONTRAfold Fast_Exp for RealT=float (cpu.py:560-580), branch-free except
+// for the x >= 0 tail, which calls expf.  A term with x >= 0 is >= 1 and the
+// posterior it feeds is clamped to 1, so that tail never shows in a result.
+__constant__ const float cf_exp_coef[6][4] = {
+    {0.0000803850f, 0.0021627428f, 0.0194708555f, 0.0588080014f},
+    {0.0013889414f, 0.0244676474f, 0.1471290604f, 0.3042757740f},
+    {0.0072335607f, 0.0906002677f, 0.3983111356f, 0.6245959221f},
+    {0.0232410351f, 0.2085645908f, 0.6906367911f, 0.8682322329f},
+    {0.0573782771f, 0.3580258429f, 0.9121133217f, 0.9793091728f},
+    {0.1199175927f, 0.4815668234f, 0.9975991939f, 0.9999505077f},
+};
+__device__ __forceinline__ float cf_fexp(float x, const float4* __restrict__ E) {
+    int seg = (x >= -5.8622823336f ? 1 : 0) + (x >= -3.8396630909f ? 1 : 0) + (x >= -2.4915033807f ? 1 : 0) +
+              (x >= -1.4805375919f ? 1 : 0) + (x >= -0.6725053211f ? 1 : 0);
+    const float4 c = E[seg];
+    float r = c.x * x;
+    r = r + c.y;
+    r = r * x;
+    r = r + c.z;
+    r = r * x;
+    r = r + c.w;
+    if (x < -9.91152f) r = 0.0f;
+    if (x >= 0.0f) r = (x > 46.052f) ? 1e20f : expf(x);
+    return r;
+}
+
+// ---------------------------------------------------------------- bitmasks
+// Partner masks: bit q of row x (x = base code of the left position, or 4 if
+// it is N/forced) says whether position q may pair with base x.  A window of
+// up to 31 consecutive q is cut out with one 64-bit shift.
+__device__ __forceinline__ uint32_t cf_window(const uint32_t* __restrict__ row, int qlo, int qhi) {
+    if (qhi < qlo) return 0u;
+    const int w = qlo >> 5, sh = qlo & 31;
+    const uint64_t v = ((uint64_t)row[w + 1] << 32) | (uint64_t)row[w];
+    const uint32_t bits = (uint32_t)(v >> sh);
+    const int n = qhi - qlo;  // <= 30
+    return bits & ((2u << n) - 1u);
+}

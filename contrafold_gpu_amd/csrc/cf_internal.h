@@ -1,0 +1,149 @@
+// cf_internal.h -- host-side state shared by the batch API (cf_api.hip) and
+// the serving engine (cf_server.hip).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cf_device.h"
+#include "../../include/contrafold_amd.h"
+
+namespace cfi {
+
+static const char* kKernelNames[] = {"inside", "exterior", "outside", "mea", "viterbi", "sample", "grad"};
+constexpr int kNumKernels = 7;
+enum { K_INSIDE = 0, K_EXTERIOR, K_OUTSIDE, K_MEA, K_VITERBI, K_SAMPLE, K_GRAD };
+
+// Caching device allocator: blocks are rounded to 2 MiB and kept on a free
+// list after release, so steady-state batches do not call hipMalloc.
+struct DevPool {
+    std::multimap<size_t, void*> free_;
+    static size_t round(size_t n) { return (n + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1); }
+    hipError_t get(void** p, size_t n) {
+        n = round(n ? n : 1);
+        auto it = free_.find(n);
+        if (it != free_.end()) {
+            *p = it->second;
+            free_.erase(it);
+            return hipSuccess;
+        }
+        hipError_t e = hipMalloc(p, n);
+        if (e != hipSuccess) {  // drop the cache and retry once
+            (void)hipGetLastError();
+            trim();
+            e = hipMalloc(p, n);
+        }
+        return e;
+    }
+    void put(void* p, size_t n) {
+        if (p) free_.emplace(round(n ? n : 1), p);
+    }
+    void trim() {
+        for (auto& kv : free_) (void)hipFree(kv.second);
+        free_.clear();
+    }
+};
+
+}  // namespace cfi
+
+struct cf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    CfParams<float>* dP32 = nullptr;
+    CfParams<double>* dP64 = nullptr;
+    uint64_t params_version = 0;
+    std::vector<int> edges{64, 128, 256, 1 << 30};
+    std::vector<int> threads{64, 128, 256, 256};
+    int lds_max = 65536;
+    int n_cu = 256;
+    cfi::DevPool pool;
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+struct Region {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+struct cf_batch {
+    cf_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;  // the context stream, or a serving stream
+    int n = 0, Lmax = 0;
+    std::vector<int> L;
+    std::vector<int64_t> charoff;
+    std::vector<int> order;
+    struct Bucket {
+        int begin, end, threads, Lmax;
+    };
+    std::vector<Bucket> buckets;
+    int64_t total_C = 0, total_T = 0, total_X = 0, total_chars = 0;
+    // inputs + inside
+    Region r_in, r_inside;
+    CfSeq* d_seqs = nullptr;
+    int* d_order = nullptr;
+    uint8_t* d_codes = nullptr;
+    uint8_t* d_masks = nullptr;
+    uint64_t* d_keys = nullptr;
+    int64_t* d_charoff = nullptr;
+    int64_t* d_soff = nullptr;
+    float* d_gin = nullptr;     // per-run small inputs: gammas (64) then grad weights (n)
+    float *FC = nullptr, *FM = nullptr, *FM1 = nullptr, *FM2 = nullptr, *FCX = nullptr, *F5 = nullptr,
+          *F5o = nullptr, *logz = nullptr;
+    // outside
+    Region r_out;
+    float *FCo = nullptr, *FMo = nullptr, *FM1o = nullptr, *FM2o = nullptr, *POST = nullptr;
+    // MEA
+    Region r_mea;
+    int mea_cap = 0, n_gamma = 0;
+    float* SC = nullptr;
+    int16_t* TB = nullptr;
+    uint8_t* mea_out = nullptr;
+    // Viterbi
+    Region r_vit;
+    double *VFC = nullptr, *VFM = nullptr, *VFM1 = nullptr, *vscore = nullptr, *logz64 = nullptr;
+    uint8_t* mfe_out = nullptr;
+    // sampling
+    Region r_smp;
+    int smp_cap = 0, n_samples = 0, stack_cap = 0, sgrid = 0;
+    uint8_t* samp_out = nullptr;
+    uint32_t* stack = nullptr;
+    // gradient
+    Region r_grad;
+    double* grad = nullptr;
+    unsigned done = 0;
+    uint64_t params_version = 0;
+    hipEvent_t ev[cfi::kNumKernels + 3] = {};  // stage marks 0..7, run start, run end
+    float last_ms = 0.0f;
+    float kms[cfi::kNumKernels] = {};
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    unsigned g_flags = 0;
+    int g_ngamma = -1, g_nsamp = -1;
+    uint64_t g_seed = 0;
+    int64_t device_bytes() const {
+        return (int64_t)(r_in.n + r_inside.n + r_out.n + r_mea.n + r_vit.n + r_smp.n + r_grad.n);
+    }
+};
+
+
+namespace cfi {
+constexpr int kMaxGamma = 64;
+int fail(cf_ctx* c, int code, const std::string& msg);
+// Pack and upload a batch on `stream` from the pinned buffer `staging`
+// (which must stay untouched until the stream has passed the upload).  With
+// sync = true the upload is waited for and `staging` may be reused at once.
+int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
+                 const uint64_t* keys, hipStream_t stream, void** staging, size_t* staging_bytes, bool sync,
+                 cf_batch** out);
+// Bytes of pinned staging batch_create needs for these sizes.
+size_t batch_staging_bytes(int n, int64_t total_residues);
+// Prepare buffers and per-run inputs, then enqueue the stages of `flags` on
+// the batch stream without waiting.  Host arrays are copied into `staging`
+// (pinned) and uploaded asynchronously.
+int batch_enqueue(cf_batch* b, unsigned flags, const float* gammas, int n_gamma, int n_samples, uint64_t seed,
+                  const float* weights, void* staging, size_t staging_bytes, bool timing, unsigned* ran);
+}  // namespace cfi

@@ -1,0 +1,1537 @@
+// cf_kernels.hip -- gfx950 kernels of the CONTRAfold fold engine.
+//
+// Work decomposition
+//   * one workgroup per sequence for the O(L^2)-cell sweeps (inside, outside +
+//     posterior, MEA, Viterbi).  A sweep walks anti-diagonals s = j - i; every
+//     cell of a diagonal depends only on earlier diagonals, so lanes take the
+//     cells of one diagonal and a barrier separates diagonals.
+//   * the FC cells of a diagonal that can pair (canonical, not forced) are
+//     compacted into a list first, so waves that run the long single-branch
+//     window (<= 496 candidates) carry pairable cells on every lane.
+//   * inside a cell, the candidates of the single-branch window are walked as
+//     set bits of per-base partner masks: a lane touches only pairable inner
+//     pairs, in the reference's (p ascending, q descending) order.
+//   * the exterior chains F5 / F5o are strictly serial per sequence; one lane
+//     per sequence runs them, 64 sequences per wave.
+//   * sampling: one lane per (sequence, sample) over a persistent grid.
+//
+// Bit-exactness: each cell folds its terms with cf_lse in exactly the order
+// of the reference's float32 engine (gpu_contrafold/cpu.py:583-895), with the
+// same left-to-right additions; nothing is reassociated and -ffp-contract=off
+// keeps every multiply-add as two rounded operations.
+#include "cf_device.h"
+
+#define WAVE 64
+
+// ------------------------------------------------------------------ LDS plan
+struct CfLds {
+    int W;  // 32-bit words per partner-mask row
+    int o_lse, o_exp, o_cs, o_jb, o_bp, o_qinfo, o_s8, o_fo8, o_rs8, o_pmask, o_dbuf, o_alist, o_cnt, o_ring, total;
+};
+__host__ __device__ inline int cf_align(int x, int a) { return (x + a - 1) / a * a; }
+__host__ __device__ inline CfLds cf_lds_plan(int Lmax, bool ring, int dbuf_elem) {
+    CfLds p;
+    p.W = ((Lmax + 2 + 31) >> 5) + 1;
+    int o = 0;
+    p.o_lse = o; o += 8 * 16;
+    p.o_exp = o; o += 8 * 16;
+    p.o_cs = o; o += 961 * 4;
+    p.o_jb = o; o += 625 * 4;
+    p.o_bp = o; o += 25 * 4;
+    o = cf_align(o, 16);
+    p.o_qinfo = o; o += (Lmax + 3) * 4;
+    p.o_s8 = o; o += Lmax + 3;
+    p.o_fo8 = o; o += Lmax + 3;
+    p.o_rs8 = o; o += Lmax + 3;
+    o = cf_align(o, 16);
+    p.o_pmask = o; o += 5 * p.W * 4;
+    o = cf_align(o, 16);
+    p.o_dbuf = o; o += (Lmax + 1) * dbuf_elem;
+    o = cf_align(o, 16);
+    p.o_alist = o; o += (Lmax + 1) * 2;
+    o = cf_align(o, 16);
+    p.o_cnt = o; o += 16;
+    p.o_ring = o;
+    if (ring) o += CF_RING * (Lmax + 1) * dbuf_elem;
+    p.total = cf_align(o, 16);
+    return p;
+}
+
+struct CfShared {
+    float4* lse;
+    float4* ex;
+    float* cs;
+    float* JB;
+    float* bp;
+    int* qinfo;
+    int8_t* s8;
+    int8_t* fo8;
+    int8_t* rs8;
+    uint32_t* pmask;
+    int W;
+    void* dbuf;
+    uint16_t* alist;
+    int* cnt;
+    void* ring;
+};
+
+__device__ inline CfShared cf_carve(unsigned char* smem, const CfLds& p) {
+    CfShared S;
+    S.lse = (float4*)(smem + p.o_lse);
+    S.ex = (float4*)(smem + p.o_exp);
+    S.cs = (float*)(smem + p.o_cs);
+    S.JB = (float*)(smem + p.o_jb);
+    S.bp = (float*)(smem + p.o_bp);
+    S.qinfo = (int*)(smem + p.o_qinfo);
+    S.s8 = (int8_t*)(smem + p.o_s8);
+    S.fo8 = (int8_t*)(smem + p.o_fo8);
+    S.rs8 = (int8_t*)(smem + p.o_rs8);
+    S.pmask = (uint32_t*)(smem + p.o_pmask);
+    S.W = p.W;
+    S.dbuf = (void*)(smem + p.o_dbuf);
+    S.alist = (uint16_t*)(smem + p.o_alist);
+    S.cnt = (int*)(smem + p.o_cnt);
+    S.ring = (void*)(smem + p.o_ring);
+    return S;
+}
+
+// Stage the per-sequence inputs and the hot parameter tables in LDS.
+__device__ void cf_stage(const CfShared& S, const CfParams<float>* __restrict__ P, const uint8_t* __restrict__ codes,
+                         const uint8_t* __restrict__ masks, const CfSeq& sd) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int L = sd.L;
+    for (int t = tid; t < 8; t += NT) S.lse[t] = make_float4(cf_lse_coef[t][0], cf_lse_coef[t][1], cf_lse_coef[t][2], cf_lse_coef[t][3]);
+    for (int t = tid; t < 6; t += NT) S.ex[t] = make_float4(cf_exp_coef[t][0], cf_exp_coef[t][1], cf_exp_coef[t][2], cf_exp_coef[t][3]);
+    for (int t = tid; t < 961; t += NT) S.cs[t] = P->t[CF_O_CS + t];
+    for (int t = tid; t < 625; t += NT) S.JB[t] = P->JB[t];
+    for (int t = tid; t < 25; t += NT) S.bp[t] = P->t[CF_O_BP + t];
+    for (int q = tid; q < L + 2; q += NT) {
+        S.s8[q] = (int8_t)codes[sd.coff + q];
+        S.fo8[q] = (masks != nullptr && masks[sd.coff + q] != 0) ? 1 : 0;
+    }
+    if (tid == 0) { S.s8[L + 2] = 4; S.fo8[L + 2] = 0; S.cnt[0] = 0; S.cnt[1] = 0; }
+    __syncthreads();
+    for (int q = tid; q < L + 2; q += NT) {
+        const int c = S.s8[q];
+        S.rs8[q] = (S.fo8[q] || c == 4) ? 4 : c;
+        S.qinfo[q] = (c << 16) | (c * 125 + S.s8[q + 1] * 5);
+    }
+    for (int w = tid; w < S.W; w += NT) {
+        uint32_t m[5] = {0u, 0u, 0u, 0u, 0u};
+        for (int bit = 0; bit < 32; ++bit) {
+            const int q = w * 32 + bit;
+            if (q > L + 1) break;
+            const int c = S.s8[q];
+            if (S.fo8[q] || c == 4) continue;
+            for (int x = 0; x < 4; ++x)
+                if (cf_canon(x, c)) m[x] |= 1u << bit;
+        }
+        for (int x = 0; x < 5; ++x) S.pmask[x * S.W + w] = m[x];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool cf_allow(const CfShared& S, int a, int b) {
+    return cf_canon(S.s8[a], S.s8[b]) && !S.fo8[a] && !S.fo8[b];
+}
+
+// ==================================================================== inside
+// FP32 inside sweep (cpu.py:595-665; gpu.py:79-168).  Writes FC, FM1, FM,
+// FM2 (the multiloop split, kept for sampling and gradients) and the exterior
+// copy FCX of FC.  F5 is produced by cf_exterior_kernel.
+template <bool RING>
+__global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
+                                                        const uint8_t* __restrict__ codes, const uint8_t* __restrict__ masks,
+                                                        const CfParams<float>* __restrict__ P, float* __restrict__ FC,
+                                                        float* __restrict__ FM, float* __restrict__ FM1,
+                                                        float* __restrict__ FM2, float* __restrict__ FCX, int Lmax) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const CfLds plan = cf_lds_plan(Lmax, RING, 4);
+    const CfShared S = cf_carve(smem, plan);
+    const CfSeq sd = seqs[order[blockIdx.x]];
+    const int L = sd.L;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    cf_stage(S, P, codes, masks, sd);
+
+    const float mb = P->t[CF_O_MB], mu = P->t[CF_O_MU], mp = P->t[CF_O_MP];
+    const float* __restrict__ T = P->t;
+    float* __restrict__ ring = (float*)S.ring;
+    float* __restrict__ fm2d = (float*)S.dbuf;
+    const int RS = Lmax + 1;
+    float* __restrict__ fc_t = FC + sd.toff;
+    float* __restrict__ fm_t = FM + sd.toff;
+    float* __restrict__ fm1_t = FM1 + sd.toff;
+    float* __restrict__ fm2_t = FM2 + sd.toff;
+    float* __restrict__ fcx_t = FCX + sd.xoff;
+    const float4* __restrict__ C = S.lse;
+
+    for (int s = 0; s <= L; ++s) {
+        const int ncell = L + 1 - s;
+        const int64_t db = cf_dbase(L, s);
+        const int slot = s % CF_RING;
+        // ---- phase 1: multiloop split, FM1, FM for every cell; collect pairable cells
+        for (int i = tid; i < ncell; i += NT) {
+            const int j = i + s;
+            const bool inner = (i > 0) && (j < L);
+            const bool alw = inner && cf_allow(S, i, j + 1);
+            float fm2 = CF_NEG;
+            if (inner && s >= 4) {
+                // FM2[i][j] = LSE_{k=i+2..j-2} FM1[i][k] + FM[k][j]  (k = i + t)
+                int64_t ia = cf_dbase(L, 2) + i;
+                int64_t ib = cf_dbase(L, s - 2) + i + 2;
+                for (int t = 2; t <= s - 2; ++t) {
+                    const float x = fm1_t[ia];
+                    const float y = fm_t[ib];
+                    const float v = (x > CF_HALF && y > CF_HALF) ? x + y : CF_NEG;
+                    fm2 = cf_lse(fm2, v, C);
+                    ia += L + 1 - t;
+                    ib -= L + 1 - (s - t - 1);
+                    ib += 1;
+                }
+            }
+            float fm1 = CF_NEG, fmv = CF_NEG;
+            if (inner && s >= 2) {
+                const int si1 = S.s8[i + 1], sj = S.s8[j];
+                if (cf_allow(S, i + 1, j)) {
+                    const float fcin = RING ? ring[((s - 2) % CF_RING) * RS + i + 1] : fc_t[cf_dbase(L, s - 2) + i + 1];
+                    if (fcin > CF_HALF) {
+                        float jA = T[CF_O_HC + cf_i2(sj, si1)];
+                        if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, S.s8[j + 1])];
+                        if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, S.s8[i])];
+                        float v = fcin + jA;
+                        v = v + mp;
+                        v = v + S.bp[cf_i2(si1, sj)];
+                        fm1 = v;
+                    }
+                }
+                const float f1n = fm1_t[cf_dbase(L, s - 1) + i + 1];
+                if (f1n > CF_HALF) fm1 = cf_lse(fm1, f1n + mu, C);
+                if (fm2 > CF_HALF) fmv = fm2;
+                const float fml = fm_t[cf_dbase(L, s - 1) + i];
+                if (fml > CF_HALF) fmv = cf_lse(fmv, fml + mu, C);
+                if (fm1 > CF_HALF) fmv = cf_lse(fmv, fm1, C);
+            }
+            fm1_t[db + i] = fm1;
+            fm_t[db + i] = fmv;
+            fm2_t[db + i] = fm2;
+            if (alw) {
+                fm2d[i] = fm2;
+                const int slotn = atomicAdd(&S.cnt[s & 1], 1);
+                S.alist[slotn] = (uint16_t)i;
+            } else {
+                fc_t[db + i] = CF_NEG;
+                if (RING) ring[slot * RS + i] = CF_NEG;
+                if (inner) fcx_t[cf_xbase(j + 1) + i - 1] = CF_NEG;
+            }
+        }
+        __syncthreads();
+        // ---- phase 2: FC for the pairable cells (hairpin, single-branch window, multiloop)
+        const int nal = S.cnt[s & 1];
+        if (tid == 0) S.cnt[(s + 1) & 1] = 0;
+        for (int idx = tid; idx < nal; idx += NT) {
+            const int i = S.alist[idx];
+            const int j = i + s;
+            const int si = S.s8[i], sj1 = S.s8[j + 1], si1 = S.s8[i + 1], sj = S.s8[j];
+            const float jB = S.JB[cf_i4(si, sj1, si1, sj)];
+            float acc = jB + T[CF_O_HP + (s < 30 ? s : 30)];
+            const float e_stack = S.bp[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
+            const float sn01 = T[CF_O_B01 + sj], sn10 = T[CF_O_B01 + si1], sn11 = T[CF_O_I11 + cf_i2(si1, sj)];
+            const int pend = min(i + CF_CMAX, j - 2);
+            int p = i, qlo = 0, sp1 = 0, rowJ = 0;
+            uint32_t bits = 0u;
+            if (p <= pend) {
+                qlo = max(p + 2, j - CF_CMAX);
+                sp1 = S.s8[p + 1];
+                rowJ = sp1 * 25 + S.s8[p];
+                bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
+            }
+            while (true) {
+                while (bits == 0u && p < pend) {
+                    ++p;
+                    qlo = max(p + 2, j - CF_CMAX + (p - i));
+                    sp1 = S.s8[p + 1];
+                    rowJ = sp1 * 25 + S.s8[p];
+                    bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
+                }
+                if (bits == 0u) break;
+                const int k = 31 - __clz(bits);
+                bits &= ~(1u << k);
+                const int q = qlo + k;
+                const int l1 = p - i, l2 = j - q;
+                const int dd = s - l1 - l2 - 2;
+                const float fc = RING ? ring[(dd % CF_RING) * RS + p + 1] : fc_t[cf_dbase(L, dd) + p + 1];
+                const int qi = S.qinfo[q];
+                float e = S.cs[l1 * 31 + l2] + S.bp[sp1 * 5 + (qi >> 16)];
+                e = e + jB;
+                e = e + S.JB[(qi & 0xffff) + rowJ];
+                const float sn = (l1 == 0 && l2 == 1) ? sn01 : (l1 == 1 && l2 == 0) ? sn10 : (l1 == 1 && l2 == 1) ? sn11 : 0.0f;
+                e = e + sn;
+                if (l1 + l2 == 0) e = e_stack;
+                const float v = (fc > CF_HALF) ? e + fc : CF_NEG;
+                acc = cf_lse(acc, v, C);
+            }
+            const float fm2 = fm2d[i];
+            if (fm2 > CF_HALF) {
+                float jA = T[CF_O_HC + cf_i2(si, sj1)];
+                if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
+                if (j > 0) jA = jA + T[CF_O_DR + cf_i3(si, sj1, sj)];
+                float v = fm2 + jA;
+                v = v + mp;
+                v = v + mb;
+                acc = cf_lse(acc, v, C);
+            }
+            fc_t[db + i] = acc;
+            if (RING) ring[slot * RS + i] = acc;
+            fcx_t[cf_xbase(j + 1) + i - 1] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// ================================================================== exterior
+// The exterior chains, one lane per sequence (cpu.py:666-677 and 684-699):
+// threads 0..63 of a block fold F5 for 64 sequences, threads 64..127 fold the
+// outside F5o for the same sequences, in the reference's order.  FC is read
+// from the column-major copy FCX, which each lane streams sequentially.
+__global__ void __launch_bounds__(128) cf_exterior_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
+                                                          int n, const uint8_t* __restrict__ codes,
+                                                          const uint8_t* __restrict__ masks,
+                                                          const CfParams<float>* __restrict__ P,
+                                                          const float* __restrict__ FCX, float* __restrict__ F5,
+                                                          float* __restrict__ F5o, float* __restrict__ logz,
+                                                          int mode) {
+    __shared__ float4 C[8];
+    __shared__ float tabs[25 + 25 + 125 + 125];
+    const int tid = threadIdx.x;
+    if (tid < 8) C[tid] = make_float4(cf_lse_coef[tid][0], cf_lse_coef[tid][1], cf_lse_coef[tid][2], cf_lse_coef[tid][3]);
+    for (int t = tid; t < 300; t += blockDim.x) {
+        float v;
+        if (t < 25) v = P->t[CF_O_BP + t];
+        else if (t < 50) v = P->t[CF_O_HC + t - 25];
+        else if (t < 175) v = P->t[CF_O_DL + t - 50];
+        else v = P->t[CF_O_DR + t - 175];
+        tabs[t] = v;
+    }
+    __syncthreads();
+    const float* bp = tabs;
+    const float* hc = tabs + 25;
+    const float* dl = tabs + 50;
+    const float* dr = tabs + 175;
+    const bool outside = tid >= 64;  // mode bit 0: F5 chain, bit 1: F5o chain
+    if (outside ? !(mode & 2) : !(mode & 1)) return;
+    const int li = blockIdx.x * 64 + (tid & 63);
+    if (li >= n) return;
+    const int b = order[li];
+    const CfSeq sd = seqs[b];
+    const int L = sd.L;
+    const uint8_t* __restrict__ s = codes + sd.coff;
+    const uint8_t* __restrict__ fo = masks ? masks + sd.coff : nullptr;
+    const float* __restrict__ fcx = FCX + sd.xoff;
+    const float eu = P->t[CF_O_EU], ep = P->t[CF_O_EP];
+    auto forced = [&](int q) -> bool { return fo != nullptr && fo[q] != 0; };
+    if (!outside) {
+        float* __restrict__ f5 = F5 + sd.coff;
+        f5[0] = 0.0f;
+        for (int j = 1; j <= L; ++j) {
+            float acc = f5[j - 1] + eu;
+            const int sj = s[j];
+            const int sj1 = s[j + 1];
+            if (!forced(j) && sj != 4) {
+                const float* __restrict__ col = fcx + cf_xbase(j);
+                for (int k = 0; k < j; ++k) {
+                    const int sk1 = s[k + 1];
+                    if (cf_canon(sk1, sj) && !forced(k + 1)) {
+                        const float fc = col[k];
+                        const float f5k = f5[k];
+                        if (fc > CF_HALF && f5k > CF_HALF) {
+                            float jA = hc[cf_i2(sj, sk1)];
+                            if (j < L) jA = jA + dl[cf_i3(sj, sk1, sj1)];
+                            if (k > 0) jA = jA + dr[cf_i3(sj, sk1, s[k])];
+                            float v = f5k + fc;
+                            v = v + ep;
+                            v = v + bp[cf_i2(sk1, sj)];
+                            v = v + jA;
+                            acc = cf_lse(acc, v, C);
+                        }
+                    }
+                }
+            }
+            f5[j] = acc;
+        }
+        f5[L + 1] = CF_NEG;
+        logz[b] = f5[L];
+    } else {
+        float* __restrict__ f5o = F5o + sd.coff;
+        for (int k = 0; k <= L + 1; ++k) f5o[k] = CF_NEG;
+        f5o[L] = 0.0f;
+        for (int j = L; j >= 1; --j) {
+            const float f5oj = f5o[j];
+            f5o[j - 1] = cf_lse(f5o[j - 1], f5oj + eu, C);
+            const int sj = s[j];
+            const int sj1 = s[j + 1];
+            if (forced(j) || sj == 4) continue;
+            const float* __restrict__ col = fcx + cf_xbase(j);
+            for (int k = 0; k < j; ++k) {
+                const int sk1 = s[k + 1];
+                if (cf_canon(sk1, sj) && !forced(k + 1)) {
+                    const float fc = col[k];
+                    if (fc > CF_HALF) {
+                        float jA = hc[cf_i2(sj, sk1)];
+                        if (j < L) jA = jA + dl[cf_i3(sj, sk1, sj1)];
+                        if (k > 0) jA = jA + dr[cf_i3(sj, sk1, s[k])];
+                        float temp = f5oj + ep;
+                        temp = temp + bp[cf_i2(sk1, sj)];
+                        temp = temp + jA;
+                        f5o[k] = cf_lse(f5o[k], temp + fc, C);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// =================================================================== outside
+// FP32 outside sweep + posterior (cpu.py:680-819), as per-cell gathers that
+// replay the reference's scatter order.  Diagonals are walked from s = L
+// down to 0; each target cell folds its sources (all on larger diagonals) in
+// the order the reference's (i ascending, j descending) sweep delivers them.
+// POST of the pair (a, b+1) is gathered in the same step as FCo[a][b]: its
+// single-branch sources are exactly FCo's, visited in reverse order.
+// POSTF is indexed by FC cell: POSTF[tri(a, b)] = P(a pairs b+1).
+template <bool RING>
+__global__ void __launch_bounds__(256) cf_outside_kernel(
+    const CfSeq* __restrict__ seqs, const int* __restrict__ order, const uint8_t* __restrict__ codes,
+    const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
+    const float* __restrict__ FM, const float* __restrict__ FM1, const float* __restrict__ F5,
+    const float* __restrict__ F5o, float* __restrict__ FCo, float* __restrict__ FMo, float* __restrict__ FM1o,
+    float* __restrict__ FM2o, float* __restrict__ POSTF, int Lmax) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const CfLds plan = cf_lds_plan(Lmax, RING, 4);
+    const CfShared S = cf_carve(smem, plan);
+    const CfSeq sd = seqs[order[blockIdx.x]];
+    const int L = sd.L;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int lane = tid & (WAVE - 1);
+    cf_stage(S, P, codes, masks, sd);
+
+    const float mb = P->t[CF_O_MB], mu = P->t[CF_O_MU], mp = P->t[CF_O_MP], ep = P->t[CF_O_EP];
+    const float* __restrict__ T = P->t;
+    float* __restrict__ ring = (float*)S.ring;
+    float* __restrict__ fmod = (float*)S.dbuf;
+    const int RS = Lmax + 1;
+    const float* __restrict__ fc_t = FC + sd.toff;
+    const float* __restrict__ fm_t = FM + sd.toff;
+    const float* __restrict__ fm1_t = FM1 + sd.toff;
+    const float* __restrict__ f5 = F5 + sd.coff;
+    const float* __restrict__ f5o = F5o + sd.coff;
+    float* __restrict__ fco_t = FCo + sd.toff;
+    float* __restrict__ fmo_t = FMo + sd.toff;
+    float* __restrict__ fm1o_t = FM1o + sd.toff;
+    float* __restrict__ fm2o_t = FM2o + sd.toff;
+    float* __restrict__ post_t = POSTF + sd.toff;
+    const float4* __restrict__ C = S.lse;
+    const float4* __restrict__ E = S.ex;
+    const float Z = f5[L];
+
+    for (int s = L; s >= 0; --s) {
+        const int ncell = L + 1 - s;
+        const int64_t db = cf_dbase(L, s);
+        const int slot = s % CF_RING;
+        // ---- phase 1: FMo for every multiloop cell; collect pairable cells
+        for (int i0 = tid - lane; i0 < ncell; i0 += NT) {
+            const int a = i0 + lane;
+            const int b = a + s;
+            const bool live = a < ncell;
+            const bool dom = live && (a > 0) && (a + 2 <= b) && (b < L);
+            const bool alw = live && (a > 0) && (b < L) && cf_allow(S, a, b + 1);
+            float fmo = CF_NEG;
+            // (i) FM2o(i', b) + FM1[i'][a] for i' = 1 .. a-2 ascending, i.e. u = a - i' descending.
+            // Lanes of a wave step u together so both reads stay contiguous.
+            const int amax = min(i0 + WAVE - 1, ncell - 1);
+            for (int u = amax - 1; u >= 2; --u) {
+                if (dom && u <= a - 1) {
+                    const int ip = a - u;
+                    const float x = fm2o_t[cf_dbase(L, s + u) + ip];
+                    const float y = fm1_t[cf_dbase(L, u) + ip];
+                    if (x > CF_HALF && y > CF_HALF) fmo = cf_lse(fmo, x + y, C);
+                }
+            }
+            // (ii) FMo[a][b+1] + mu
+            if (dom && b + 1 < L) {
+                const float x = fmo_t[cf_dbase(L, s + 1) + a];
+                if (x > CF_HALF) fmo = cf_lse(fmo, x + mu, C);
+            }
+            if (live) {
+                fmo_t[db + a] = fmo;
+                fmod[a] = fmo;
+                if (alw) {
+                    const int slotn = atomicAdd(&S.cnt[s & 1], 1);
+                    S.alist[slotn] = (uint16_t)a;
+                } else {
+                    fco_t[db + a] = CF_NEG;
+                    if (RING) ring[slot * RS + a] = CF_NEG;
+                    post_t[db + a] = 0.0f;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- phase 2: FCo and POST for pairable cells
+        const int nal = S.cnt[s & 1];
+        if (tid == 0) S.cnt[(s + 1) & 1] = 0;
+        for (int idx = tid; idx < nal; idx += NT) {
+            const int a = S.alist[idx];
+            const int b = a + s;
+            const int sa = S.s8[a], sb1 = S.s8[b + 1], sam = S.s8[a - 1], sb2 = S.s8[b + 2];
+            const float bpab = S.bp[cf_i2(sa, sb1)];
+            const float fcab = fc_t[db + a];
+            // jA of branch (a, b+1) seen from the enclosing loop (exterior or multiloop)
+            float jAx = T[CF_O_HC + cf_i2(sb1, sa)];
+            if (b + 1 < L) jAx = jAx + T[CF_O_DL + cf_i3(sb1, sa, sb2)];
+            if (a - 1 > 0) jAx = jAx + T[CF_O_DR + cf_i3(sb1, sa, sam)];
+            const float jB2 = S.JB[cf_i4(sb1, sa, sb2, sam)];
+            const float e_stack = bpab + T[CF_O_STACK + cf_i4(sam, sb2, sa, sb1)];
+            const bool fm1dom = (a - 1 > 0) && (a + 1 <= b + 1) && (b + 1 < L);
+            const float fm1o_src = fm1dom ? fm1o_t[cf_dbase(L, s + 2) + a - 1] : CF_NEG;
+            const int lmax = min(CF_CMAX, a - 2);
+            const int qbase = b + 2;
+            // ---------- FCo[a][b]
+            float acc = CF_NEG;
+            const float f5a = f5[a - 1];
+            if (f5a > CF_HALF) {
+                float temp = f5o[b + 1] + ep;
+                temp = temp + bpab;
+                temp = temp + jAx;
+                acc = temp + f5a;
+            }
+            for (int l1 = lmax; l1 >= 0; --l1) {
+                const int ip = a - 1 - l1;
+                uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
+                const int sip = S.s8[ip], sip1 = S.s8[ip + 1];
+                bool fm1_pending = (l1 == 0);
+                while (true) {
+                    if (fm1_pending && (bits & ~1u) == 0u) {
+                        // the FM1o source of iteration (a-1, b+1) precedes its stack term
+                        fm1_pending = false;
+                        if (fm1o_src > CF_HALF) {
+                            float v = fm1o_src + jAx;
+                            v = v + mp;
+                            v = v + bpab;
+                            acc = cf_lse(acc, v, C);
+                        }
+                    }
+                    if (bits == 0u) break;
+                    const int k = 31 - __clz(bits);
+                    bits &= ~(1u << k);
+                    const int jp = qbase + k - 1;  // outer cell (ip, jp)
+                    const int l2 = k;
+                    const int sl = (s + l1 + l2 + 2) % CF_RING;
+                    const float fco = RING ? ring[sl * RS + ip] : fco_t[cf_dbase(L, s + l1 + l2 + 2) + ip];
+                    const int sjp = S.s8[jp];
+                    float e = S.cs[l1 * 31 + l2] + bpab;
+                    e = e + S.JB[cf_i4(sip, S.s8[jp + 1], sip1, sjp)];
+                    e = e + jB2;
+                    const float sn = (l1 == 0 && l2 == 1) ? T[CF_O_B01 + sjp]
+                                   : (l1 == 1 && l2 == 0) ? T[CF_O_B01 + sip1]
+                                   : (l1 == 1 && l2 == 1) ? T[CF_O_I11 + cf_i2(sip1, sjp)] : 0.0f;
+                    e = e + sn;
+                    if (l1 + l2 == 0) e = e_stack;
+                    const float v = (fco > CF_HALF) ? fco + e : CF_NEG;
+                    acc = cf_lse(acc, v, C);
+                }
+            }
+            fco_t[db + a] = acc;
+            if (RING) ring[slot * RS + a] = acc;
+            // ---------- POST(a, b+1): same sources, reverse order, then exterior
+            float post = 0.0f;
+            if (fcab > CF_HALF) {
+                for (int l1 = 0; l1 <= lmax; ++l1) {
+                    const int ip = a - 1 - l1;
+                    uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
+                    const int sip = S.s8[ip], sip1 = S.s8[ip + 1];
+                    bool fm1_pending = (l1 == 0);
+                    while (true) {
+                        if (fm1_pending && (bits & 1u) == 0u) {
+                            fm1_pending = false;
+                            if (fm1o_src > CF_HALF) {
+                                float v = fm1o_src + fcab;
+                                v = v + jAx;
+                                v = v + mp;
+                                v = v + bpab;
+                                v = v - Z;
+                                post = post + cf_fexp(v, E);
+                            }
+                        }
+                        if (bits == 0u) break;
+                        const int k = __ffs(bits) - 1;
+                        bits &= bits - 1u;
+                        const int jp = qbase + k - 1;
+                        const int l2 = k;
+                        const int sl = (s + l1 + l2 + 2) % CF_RING;
+                        const float fco = RING ? ring[sl * RS + ip] : fco_t[cf_dbase(L, s + l1 + l2 + 2) + ip];
+                        const float outside = fco - Z;
+                        if (outside > CF_HALF) {
+                            const int sjp = S.s8[jp];
+                            float e;
+                            if (l1 + l2 == 0) {
+                                e = outside + bpab;
+                                e = e + T[CF_O_STACK + cf_i4(sam, sb2, sa, sb1)];
+                                e = e + fcab;
+                            } else {
+                                e = outside + S.JB[cf_i4(sip, S.s8[jp + 1], sip1, sjp)];
+                                e = e + S.cs[l1 * 31 + l2];
+                                e = e + fcab;
+                                e = e + bpab;
+                                e = e + jB2;
+                                const float sn = (l1 == 0 && l2 == 1) ? T[CF_O_B01 + sjp]
+                                               : (l1 == 1 && l2 == 0) ? T[CF_O_B01 + sip1]
+                                               : (l1 == 1 && l2 == 1) ? T[CF_O_I11 + cf_i2(sip1, sjp)] : 0.0f;
+                                e = e + sn;
+                            }
+                            post = post + cf_fexp(e, E);
+                        }
+                        if (fm1_pending && k == 0) {
+                            fm1_pending = false;
+                            if (fm1o_src > CF_HALF) {
+                                float v = fm1o_src + fcab;
+                                v = v + jAx;
+                                v = v + mp;
+                                v = v + bpab;
+                                v = v - Z;
+                                post = post + cf_fexp(v, E);
+                            }
+                        }
+                    }
+                }
+                const float outx = f5o[b + 1] - Z;
+                if (outx > CF_HALF && f5a > CF_HALF) {
+                    float v = outx + f5a;
+                    v = v + fcab;
+                    v = v + ep;
+                    v = v + bpab;
+                    v = v + jAx;
+                    post = post + cf_fexp(v, E);
+                }
+                post = post < 0.0f ? 0.0f : (post > 1.0f ? 1.0f : post);
+            }
+            post_t[db + a] = post;
+        }
+        __syncthreads();
+        // ---- phase 3: FM2o and FM1o for every multiloop cell
+        for (int i0 = tid - lane; i0 < ncell; i0 += NT) {
+            const int a = i0 + lane;
+            const int b = a + s;
+            const bool live = a < ncell;
+            const bool dom = live && (a > 0) && (a + 2 <= b) && (b < L);
+            float fm2o = CF_NEG, f1 = CF_NEG;
+            if (dom) {
+                fm2o = fmod[a];
+                if (cf_allow(S, a, b + 1)) {
+                    const float fco = RING ? ring[slot * RS + a] : fco_t[db + a];
+                    if (fco > CF_HALF) {
+                        const int sa = S.s8[a], sb1 = S.s8[b + 1];
+                        float jA = T[CF_O_HC + cf_i2(sa, sb1)];
+                        if (a < L) jA = jA + T[CF_O_DL + cf_i3(sa, sb1, S.s8[a + 1])];
+                        if (b > 0) jA = jA + T[CF_O_DR + cf_i3(sa, sb1, S.s8[b])];
+                        float v = fco + jA;
+                        v = v + mp;
+                        v = v + mb;
+                        fm2o = cf_lse(fm2o, v, C);
+                    }
+                }
+                // (i) FM1o[a-1][b] + mu
+                if (a - 1 > 0 && a + 1 <= b) {
+                    const float x = fm1o_t[cf_dbase(L, s + 1) + a - 1];
+                    if (x > CF_HALF) f1 = cf_lse(f1, x + mu, C);
+                }
+            }
+            // (ii) FM2o(a, b+v) + FM[b][b+v] for v = L-1-b .. 2 descending (lanes step v together)
+            const int vmax = L - 1 - (i0 + s);
+            for (int v = vmax; v >= 2; --v) {
+                if (dom && v <= L - 1 - b) {
+                    const float x = fm2o_t[cf_dbase(L, s + v) + a];
+                    const float y = fm_t[cf_dbase(L, v) + b];
+                    if (x > CF_HALF && y > CF_HALF) f1 = cf_lse(f1, x + y, C);
+                }
+            }
+            if (dom) {
+                const float fmo = fmod[a];
+                if (fmo > CF_HALF) f1 = cf_lse(f1, fmo, C);
+            }
+            if (live) {
+                fm2o_t[db + a] = fm2o;
+                fm1o_t[db + a] = f1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ======================================================================= MEA
+// Maximum-expected-accuracy decoding (cpu.py:823-895).  Max-plus values are
+// exact, so the strict '>' scan over the candidate list equals a parallel
+// argmax with lowest-index ties; here each lane still scans its own cell in
+// order, which is both exact and simple.  Output: dot-bracket bytes.
+__global__ void __launch_bounds__(256) cf_mea_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
+                                                     int nlaunch, const uint8_t* __restrict__ codes,
+                                                     const uint8_t* __restrict__ masks, const float* __restrict__ POSTF,
+                                                     const float* __restrict__ gammas, float* __restrict__ SC,
+                                                     int16_t* __restrict__ TB, int64_t tab_stride,
+                                                     uint8_t* __restrict__ out, const int64_t* __restrict__ out_off,
+                                                     int64_t out_stride, int Lmax) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* up = (float*)smem;                          // L+2
+    int8_t* s8 = (int8_t*)(up + (Lmax + 2));           // L+2
+    int8_t* fo8 = s8 + (Lmax + 2);                     // L+2
+    int* qs = (int*)(((uintptr_t)(fo8 + Lmax + 2) + 15) & ~(uintptr_t)15);  // traceback stack 2*(L+2)
+    const int li = blockIdx.x % nlaunch;
+    const int g = blockIdx.x / nlaunch;
+    const int bseq = order[li];
+    const CfSeq sd = seqs[bseq];
+    const int L = sd.L;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const float* __restrict__ post = POSTF + sd.toff;
+    float* __restrict__ sc = SC + g * tab_stride + sd.toff;
+    int16_t* __restrict__ tb = TB + g * tab_stride + sd.toff;
+    for (int q = tid; q < L + 2; q += NT) {
+        s8[q] = (int8_t)codes[sd.coff + q];
+        fo8[q] = (masks != nullptr && masks[sd.coff + q] != 0) ? 1 : 0;
+    }
+    const float gam = gammas[g];
+    const float two_g = 2.0f * gam;
+    // up[i] = (1 - column sum - row sum) / (2 gamma); POST(x, y) = POSTF[tri(x, y-1)]
+    for (int i = 1 + tid; i <= L; i += NT) {
+        float u = 1.0f;
+        for (int j = 1; j < i; ++j) u -= post[cf_tri(L, j, i - 1)];
+        for (int j = i + 1; j <= L; ++j) u -= post[cf_tri(L, i, j - 1)];
+        up[i] = u / two_g;
+    }
+    __syncthreads();
+    for (int s = 0; s <= L; ++s) {
+        const int64_t db = cf_dbase(L, s);
+        for (int i = tid; i <= L - s; i += NT) {
+            const int j = i + s;
+            float best = -1.0f;
+            int t = -1;
+            if (s == 0) {
+                best = 0.0f;
+                t = 0;
+            } else {
+                float v = up[i + 1] + sc[cf_dbase(L, s - 1) + i + 1];
+                if (v > best) { best = v; t = 1; }
+                v = up[j] + sc[cf_dbase(L, s - 1) + i];
+                if (v > best) { best = v; t = 2; }
+                if (s >= 2) {
+                    if (cf_canon(s8[i + 1], s8[j]) && !fo8[i + 1] && !fo8[j]) {
+                        v = post[cf_dbase(L, s - 2) + i + 1] + sc[cf_dbase(L, s - 2) + i + 1];
+                        if (v > best) { best = v; t = 3; }
+                    }
+                    int64_t ia = cf_dbase(L, 1) + i;
+                    int64_t ib = cf_dbase(L, s - 1) + i + 1;
+                    for (int d = 1; d < s; ++d) {
+                        v = sc[ia] + sc[ib];
+                        if (v > best) { best = v; t = i + d + 4; }
+                        ia += L + 1 - d;
+                        ib -= L + 1 - (s - d - 1);
+                        ib += 1;
+                    }
+                }
+            }
+            sc[db + i] = best;
+            tb[db + i] = (int16_t)t;
+        }
+        __syncthreads();
+    }
+    // traceback (cpu.py:873-895), dot-bracket output
+    uint8_t* __restrict__ o = out + g * out_stride + out_off[bseq];
+    for (int q = tid; q < L; q += NT) o[q] = '.';
+    __syncthreads();
+    if (tid == 0) {
+        int sp = 0;
+        qs[0] = 0;
+        qs[1] = L;
+        sp = 1;
+        while (sp > 0) {
+            --sp;
+            const int i = qs[2 * sp], j = qs[2 * sp + 1];
+            const int t = tb[cf_tri(L, i, j)];
+            if (t <= 0) continue;
+            if (t == 1) { qs[2 * sp] = i + 1; qs[2 * sp + 1] = j; ++sp; }
+            else if (t == 2) { qs[2 * sp] = i; qs[2 * sp + 1] = j - 1; ++sp; }
+            else if (t == 3) {
+                o[i] = '(';
+                o[j - 1] = ')';
+                qs[2 * sp] = i + 1; qs[2 * sp + 1] = j - 1; ++sp;
+            } else {
+                const int k = t - 4;
+                qs[2 * sp] = i; qs[2 * sp + 1] = k; ++sp;
+                qs[2 * sp] = k; qs[2 * sp + 1] = j; ++sp;
+            }
+        }
+    }
+}
+
+// =================================================================== FP64
+// One FP64 sweep kernel, two semirings:
+//  * LSE = false: Viterbi max-plus recurrence (cpu.py:296-394) and the
+//    EPS = 1e-6 first-match traceback (cpu.py:396-502).  Max is exact, so the
+//    F5 max over k is reduced in parallel; the traceback is one lane replaying
+//    the reference's candidate order.
+//  * LSE = true: the FP64 inside pass with exact log1p/exp (cpu.py:154-275),
+//    terms folded in the reference's order; writes logZ = F5[L].
+__device__ __forceinline__ double cf_lse64(double a, double b) {
+    if (a < CF_HALF64) return b;
+    if (b < CF_HALF64) return a;
+    if (a > b) return a + log1p(exp(b - a));
+    return b + log1p(exp(a - b));
+}
+template <bool LSE>
+__device__ __forceinline__ void cf_acc(double& acc, double v) {
+    if (LSE) acc = cf_lse64(acc, v);
+    else if (v > acc) acc = v;
+}
+template <bool LSE>
+__global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
+                                                         const uint8_t* __restrict__ codes,
+                                                         const uint8_t* __restrict__ masks,
+                                                         const CfParams<double>* __restrict__ P,
+                                                         double* __restrict__ FC, double* __restrict__ FM,
+                                                         double* __restrict__ FM1, double* __restrict__ score_out,
+                                                         uint8_t* __restrict__ out, const int64_t* __restrict__ out_off,
+                                                         int Lmax) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* f5 = (double*)smem;                          // L+2
+    double* red = f5 + (Lmax + 2);                       // 8 wave partials
+    int8_t* s8 = (int8_t*)(red + 8);                     // L+3
+    int8_t* fo8 = s8 + (Lmax + 3);                       // L+3
+    uint32_t* st = (uint32_t*)(((uintptr_t)(fo8 + Lmax + 3) + 15) & ~(uintptr_t)15);  // 4L+16
+    const int bseq = order[blockIdx.x];
+    const CfSeq sd = seqs[bseq];
+    const int L = sd.L;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    for (int q = tid; q < L + 2; q += NT) {
+        s8[q] = (int8_t)codes[sd.coff + q];
+        fo8[q] = (masks != nullptr && masks[sd.coff + q] != 0) ? 1 : 0;
+    }
+    if (tid == 0) { s8[L + 2] = 4; fo8[L + 2] = 0; }
+    __syncthreads();
+    const double* __restrict__ T = P->t;
+    const double mb = T[CF_O_MB], mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
+    double* __restrict__ fc_t = FC + sd.toff;
+    double* __restrict__ fm_t = FM + sd.toff;
+    double* __restrict__ fm1_t = FM1 + sd.toff;
+    auto allow = [&](int a, int b) -> bool { return cf_canon(s8[a], s8[b]) && !fo8[a] && !fo8[b]; };
+
+    for (int s = 0; s <= L; ++s) {
+        const int64_t db = cf_dbase(L, s);
+        for (int i = tid; i <= L - s; i += NT) {
+            const int j = i + s;
+            const bool inner = (i > 0) && (j < L);
+            double fm2 = CF_NEG64;
+            if (inner && s >= 4) {
+                int64_t ia = cf_dbase(L, 2) + i;
+                int64_t ib = cf_dbase(L, s - 2) + i + 2;
+                for (int t = 2; t <= s - 2; ++t) {
+                    const double x = fm1_t[ia], y = fm_t[ib];
+                    if (x > CF_HALF64 && y > CF_HALF64) cf_acc<LSE>(fm2, x + y);
+                    ia += L + 1 - t;
+                    ib -= L + 1 - (s - t - 1);
+                    ib += 1;
+                }
+            }
+            double fcv = CF_NEG64;
+            if (inner && allow(i, j + 1)) {
+                const int si = s8[i], sj1 = s8[j + 1], si1 = s8[i + 1], sj = s8[j];
+                const double jB = P->JB[cf_i4(si, sj1, si1, sj)];
+                double best = jB + T[CF_O_HP + (s < 30 ? s : 30)];
+                const int pmax = min(i + CF_CMAX, j);
+                for (int p = i; p <= pmax; ++p) {
+                    const int l1 = p - i;
+                    const int qmin = max(p + 2, p - i + j - CF_CMAX);
+                    for (int q = j; q >= qmin; --q) {
+                        if (!allow(p + 1, q)) continue;
+                        const double fc = fc_t[cf_tri(L, p + 1, q - 1)];
+                        if (!(fc > CF_HALF64)) continue;
+                        const int l2 = j - q;
+                        double e;
+                        if (l1 == 0 && l2 == 0) {
+                            e = T[CF_O_BP + cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
+                        } else {
+                            const double jB2 = P->JB[cf_i4(s8[q], s8[p + 1], s8[q + 1], s8[p])];
+                            double sn = 0.0;
+                            if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
+                            else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
+                            else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
+                            e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s8[p + 1], s8[q])];
+                            e = e + jB;
+                            e = e + jB2;
+                            e = e + sn;
+                        }
+                        cf_acc<LSE>(best, e + fc);
+                    }
+                }
+                if (fm2 > CF_HALF64) {
+                    double jA = T[CF_O_HC + cf_i2(si, sj1)];
+                    if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
+                    if (j > 0) jA = jA + T[CF_O_DR + cf_i3(si, sj1, sj)];
+                    double v = fm2 + jA;
+                    v = v + mp;
+                    v = v + mb;
+                    cf_acc<LSE>(best, v);
+                }
+                fcv = best;
+            }
+            double fm1 = CF_NEG64, fmv = CF_NEG64;
+            if (inner && s >= 2) {
+                const int si1 = s8[i + 1], sj = s8[j];
+                if (allow(i + 1, j)) {
+                    const double fcin = fc_t[cf_dbase(L, s - 2) + i + 1];
+                    if (fcin > CF_HALF64) {
+                        double jA = T[CF_O_HC + cf_i2(sj, si1)];
+                        if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, s8[j + 1])];
+                        if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, s8[i])];
+                        double v = fcin + jA;
+                        v = v + mp;
+                        v = v + T[CF_O_BP + cf_i2(si1, sj)];
+                        cf_acc<LSE>(fm1, v);
+                    }
+                }
+                const double f1n = fm1_t[cf_dbase(L, s - 1) + i + 1];
+                if (f1n > CF_HALF64) cf_acc<LSE>(fm1, f1n + mu);
+                if (fm2 > CF_HALF64) cf_acc<LSE>(fmv, fm2);
+                const double fml = fm_t[cf_dbase(L, s - 1) + i];
+                if (fml > CF_HALF64) cf_acc<LSE>(fmv, fml + mu);
+                if (fm1 > CF_HALF64) cf_acc<LSE>(fmv, fm1);
+            }
+            fc_t[db + i] = fcv;
+            fm1_t[db + i] = fm1;
+            fm_t[db + i] = fmv;
+        }
+        __syncthreads();
+    }
+    if (LSE) {
+        // F5 chain (cpu.py:259-275), one lane, reference order
+        if (tid == 0) {
+            f5[0] = 0.0;
+            for (int j = 1; j <= L; ++j) {
+                double acc = f5[j - 1] + eu;
+                const int sj = s8[j];
+                for (int k = 0; k < j; ++k) {
+                    if (!allow(k + 1, j)) continue;
+                    const double fc = fc_t[cf_tri(L, k + 1, j - 1)];
+                    if (!(fc > CF_HALF64) || !(f5[k] > CF_HALF64)) continue;
+                    const int sk1 = s8[k + 1];
+                    double jA = T[CF_O_HC + cf_i2(sj, sk1)];
+                    if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s8[j + 1])];
+                    if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s8[k])];
+                    double v = f5[k] + fc;
+                    v = v + ep;
+                    v = v + T[CF_O_BP + cf_i2(sk1, sj)];
+                    v = v + jA;
+                    acc = cf_lse64(acc, v);
+                }
+                f5[j] = acc;
+            }
+            score_out[bseq] = f5[L];
+        }
+        return;
+    }
+    // F5 (cpu.py:381-394): max over k reduced across the block
+    if (tid == 0) f5[0] = 0.0;
+    __syncthreads();
+    const int nw = (NT + WAVE - 1) / WAVE;
+    for (int j = 1; j <= L; ++j) {
+        double best = CF_NEG64;
+        const int sj = s8[j];
+        for (int k = tid; k < j; k += NT) {
+            if (allow(k + 1, j) && f5[k] > CF_HALF64) {
+                const double fc = fc_t[cf_tri(L, k + 1, j - 1)];
+                if (fc > CF_HALF64) {
+                    const int sk1 = s8[k + 1];
+                    double jA = T[CF_O_HC + cf_i2(sj, sk1)];
+                    if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s8[j + 1])];
+                    if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s8[k])];
+                    double v = f5[k] + fc;
+                    v = v + ep;
+                    v = v + T[CF_O_BP + cf_i2(sk1, sj)];
+                    v = v + jA;
+                    if (v > best) best = v;
+                }
+            }
+        }
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            const double o = __shfl_xor(best, off);
+            best = o > best ? o : best;
+        }
+        if ((tid & (WAVE - 1)) == 0) red[tid / WAVE] = best;
+        __syncthreads();
+        if (tid == 0) {
+            double m = f5[j - 1] + eu;
+            for (int w = 0; w < nw; ++w)
+                if (red[w] > m) m = red[w];
+            f5[j] = m;
+        }
+        __syncthreads();
+    }
+    // traceback
+    uint8_t* __restrict__ o = out + out_off[bseq];
+    for (int q = tid; q < L; q += NT) o[q] = '.';
+    __syncthreads();
+    if (tid != 0) return;
+    if (score_out) score_out[bseq] = f5[L];
+    const double EPS = 1e-6;
+    auto push = [&](int& sp, int typ, int i, int j) { st[sp++] = (uint32_t)typ | ((uint32_t)i << 2) | ((uint32_t)j << 17); };
+    auto hc = [&](int a, int b) { return T[CF_O_HC + cf_i2(a, b)]; };
+    int sp = 0;
+    push(sp, 0, 0, L);
+    while (sp > 0) {
+        --sp;
+        const uint32_t e0 = st[sp];
+        const int typ = e0 & 3, i = (e0 >> 2) & 0x7fff, j = (e0 >> 17) & 0x7fff;
+        if (typ == 0) {
+            if (j == 0) continue;
+            const double val = f5[j];
+            bool done = false;
+            if (f5[j - 1] > CF_HALF64 && fabs(f5[j - 1] + eu - val) <= EPS) { push(sp, 0, 0, j - 1); done = true; }
+            if (!done) {
+                for (int k = 0; k < j; ++k) {
+                    if (allow(k + 1, j) && f5[k] > CF_HALF64) {
+                        const double fc = fc_t[cf_tri(L, k + 1, j - 1)];
+                        if (!(fc > CF_HALF64)) continue;
+                        const int sj = s8[j], sk1 = s8[k + 1];
+                        double jA = hc(sj, sk1);
+                        if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s8[j + 1])];
+                        if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s8[k])];
+                        double v = f5[k] + fc;
+                        v = v + ep;
+                        v = v + T[CF_O_BP + cf_i2(sk1, sj)];
+                        v = v + jA;
+                        if (fabs(v - val) <= EPS) {
+                            o[k] = '(';
+                            o[j - 1] = ')';
+                            push(sp, 1, k + 1, j - 1);
+                            push(sp, 0, 0, k);
+                            break;
+                        }
+                    }
+                }
+            }
+        } else if (typ == 1) {
+            const double val = fc_t[cf_tri(L, i, j)];
+            const int si = s8[i], sj1 = s8[j + 1], si1 = s8[i + 1], sj = s8[j];
+            const double jB = P->JB[cf_i4(si, sj1, si1, sj)];
+            const int d = j - i;
+            bool done = fabs(jB + T[CF_O_HP + (d <= 30 ? d : 30)] - val) <= EPS;
+            if (!done) {
+                const int pmax = min(i + CF_CMAX, j);
+                for (int p = i; p <= pmax && !done; ++p) {
+                    const int l1 = p - i;
+                    const int qmin = max(p + 2, p - i + j - CF_CMAX);
+                    for (int q = j; q >= qmin; --q) {
+                        if (!allow(p + 1, q)) continue;
+                        const double fc = fc_t[cf_tri(L, p + 1, q - 1)];
+                        if (!(fc > CF_HALF64)) continue;
+                        const int l2 = j - q;
+                        double e;
+                        if (l1 == 0 && l2 == 0) {
+                            e = T[CF_O_BP + cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
+                        } else {
+                            const double jB2 = P->JB[cf_i4(s8[q], s8[p + 1], s8[q + 1], s8[p])];
+                            double sn = 0.0;
+                            if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
+                            else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
+                            else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
+                            e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s8[p + 1], s8[q])];
+                            e = e + jB;
+                            e = e + jB2;
+                            e = e + sn;
+                        }
+                        if (fabs(e + fc - val) <= EPS) {
+                            o[p] = '(';
+                            o[q - 1] = ')';
+                            push(sp, 1, p + 1, q - 1);
+                            done = true;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (!done) {
+                double jA = hc(si, sj1);
+                if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
+                if (j > 0) jA = jA + T[CF_O_DR + cf_i3(si, sj1, sj)];
+                for (int k = i + 1; k < j; ++k) {
+                    const double x = fm1_t[cf_tri(L, i, k)], y = fm_t[cf_tri(L, k, j)];
+                    if (x > CF_HALF64 && y > CF_HALF64) {
+                        double v = x + y;
+                        v = v + jA;
+                        v = v + mp;
+                        v = v + mb;
+                        if (fabs(v - val) <= EPS) {
+                            push(sp, 3, i, k);
+                            push(sp, 2, k, j);
+                            break;
+                        }
+                    }
+                }
+            }
+        } else if (typ == 2) {
+            const double val = fm_t[cf_tri(L, i, j)];
+            bool done = false;
+            for (int k = i + 1; k < j; ++k) {
+                const double x = fm1_t[cf_tri(L, i, k)], y = fm_t[cf_tri(L, k, j)];
+                if (x > CF_HALF64 && y > CF_HALF64 && fabs(x + y - val) <= EPS) {
+                    push(sp, 3, i, k);
+                    push(sp, 2, k, j);
+                    done = true;
+                    break;
+                }
+            }
+            if (!done) {
+                const double x = fm_t[cf_tri(L, i, j - 1)];
+                if (x > CF_HALF64 && fabs(x + mu - val) <= EPS) { push(sp, 2, i, j - 1); done = true; }
+            }
+            if (!done) {
+                const double x = fm1_t[cf_tri(L, i, j)];
+                if (x > CF_HALF64 && fabs(x - val) <= EPS) push(sp, 3, i, j);
+            }
+        } else {
+            const double val = fm1_t[cf_tri(L, i, j)];
+            bool done = false;
+            if (allow(i + 1, j)) {
+                const double fc = fc_t[cf_tri(L, i + 1, j - 1)];
+                if (fc > CF_HALF64) {
+                    const int si1 = s8[i + 1], sj = s8[j];
+                    double jA = hc(sj, si1);
+                    if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, s8[j + 1])];
+                    if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, s8[i])];
+                    double v = fc + jA;
+                    v = v + mp;
+                    v = v + T[CF_O_BP + cf_i2(si1, sj)];
+                    if (fabs(v - val) <= EPS) {
+                        o[i] = '(';
+                        o[j - 1] = ')';
+                        push(sp, 1, i + 1, j - 1);
+                        done = true;
+                    }
+                }
+            }
+            if (!done) {
+                const double x = fm1_t[cf_tri(L, i + 1, j)];
+                if (x > CF_HALF64 && fabs(x + mu - val) <= EPS) push(sp, 3, i + 1, j);
+            }
+        }
+    }
+}
+
+// ================================================================== sampling
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter-based, so each
+// (sequence key, sample, draw) has its own stream, independent of batch order.
+__device__ __forceinline__ uint4 cf_philox(uint4 c, uint2 k) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+struct CfRng {
+    uint2 key;
+    uint32_t sample, seqk, ctr;
+    uint4 buf;
+    int have;
+    __device__ float next() {
+        if (have == 0) {
+            buf = cf_philox(make_uint4(sample, ctr, seqk, 0x5eedu), key);
+            ++ctr;
+            have = 4;
+        }
+        const uint32_t x = (have == 4) ? buf.x : (have == 3) ? buf.y : (have == 2) ? buf.z : buf.w;
+        --have;
+        return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f);  // (0, 1)
+    }
+};
+
+// One Boltzmann traceback per lane (the reference's state machine,
+// gpu.py:220-477, with candidate order unchanged).  Each state draws one
+// uniform r and walks its candidates' probabilities exp(w - total), where
+// total is the state's own inside value; the multiloop options are grouped
+// through the stored split FM2 and resolved with a second draw.  When FP32
+// round-off leaves r above the running sum the last candidate seen is taken.
+__global__ void __launch_bounds__(64) cf_sample_kernel(
+    const CfSeq* __restrict__ seqs, const int* __restrict__ sidx, int nseq, int nsamp, uint64_t seed,
+    const uint64_t* __restrict__ keys, const uint8_t* __restrict__ codes, const uint8_t* __restrict__ masks,
+    const CfParams<float>* __restrict__ P, const float* __restrict__ FC, const float* __restrict__ FM,
+    const float* __restrict__ FM1, const float* __restrict__ FM2, const float* __restrict__ F5,
+    uint32_t* __restrict__ stack, int stack_cap, uint8_t* __restrict__ out, const int64_t* __restrict__ out_off) {
+    const int64_t total = (int64_t)nseq * nsamp;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gsz = (int64_t)gridDim.x * blockDim.x;
+    uint32_t* __restrict__ st = stack + gid * stack_cap;
+    const float* __restrict__ T = P->t;
+    const float mb = T[CF_O_MB], mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
+    for (int64_t task = gid; task < total; task += gsz) {
+        const int li = (int)(task / nsamp);
+        const int smp = (int)(task % nsamp);
+        const int b = sidx[li];
+        const CfSeq sd = seqs[b];
+        const int L = sd.L;
+        const uint8_t* __restrict__ s = codes + sd.coff;
+        const uint8_t* __restrict__ fo = masks ? masks + sd.coff : nullptr;
+        auto allow = [&](int a, int c) -> bool {
+            return cf_canon(s[a], s[c]) && (fo == nullptr || (fo[a] == 0 && fo[c] == 0));
+        };
+        const float* __restrict__ fc_t = FC + sd.toff;
+        const float* __restrict__ fm_t = FM + sd.toff;
+        const float* __restrict__ fm1_t = FM1 + sd.toff;
+        const float* __restrict__ fm2_t = FM2 + sd.toff;
+        const float* __restrict__ f5 = F5 + sd.coff;
+        uint8_t* __restrict__ o = out + out_off[b] + (int64_t)smp * L;
+        for (int q = 0; q < L; ++q) o[q] = '.';
+        CfRng rng;
+        const uint64_t key = keys ? keys[b] : (uint64_t)b;
+        rng.key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32) ^ (uint32_t)(key >> 32));
+        rng.seqk = (uint32_t)key;
+        rng.sample = (uint32_t)smp;
+        rng.ctr = 0;
+        rng.have = 0;
+        int sp = 0;
+        auto push = [&](int typ, int i, int j) {
+            if (sp < stack_cap) st[sp++] = (uint32_t)typ | ((uint32_t)i << 2) | ((uint32_t)j << 17);
+        };
+        push(0, 0, L);
+        while (sp > 0) {
+            --sp;
+            const uint32_t e0 = st[sp];
+            const int typ = e0 & 3, i = (e0 >> 2) & 0x7fff, j = (e0 >> 17) & 0x7fff;
+            if (typ == 0) {  // F5(j): j unpaired, or pair (k+1, j) closing an exterior helix
+                if (j == 0) continue;
+                const float tot = f5[j];
+                const float r = rng.next();
+                float c = expf(f5[j - 1] + eu - tot);
+                int pick = -1;
+                if (!(r <= c)) {
+                    const int sj = s[j];
+                    for (int k = 0; k < j; ++k) {
+                        if (!allow(k + 1, j)) continue;
+                        const float fc = fc_t[cf_tri(L, k + 1, j - 1)];
+                        if (!(fc > CF_HALF) || !(f5[k] > CF_HALF)) continue;
+                        const int sk1 = s[k + 1];
+                        float jA = T[CF_O_HC + cf_i2(sj, sk1)];
+                        if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s[j + 1])];
+                        if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s[k])];
+                        const float w = f5[k] + fc + ep + T[CF_O_BP + cf_i2(sk1, sj)] + jA;
+                        c += expf(w - tot);
+                        pick = k;
+                        if (r <= c) break;
+                    }
+                }
+                if (pick < 0) {
+                    push(0, 0, j - 1);
+                } else {
+                    o[pick] = '(';
+                    o[j - 1] = ')';
+                    push(1, pick + 1, j - 1);
+                    push(0, 0, pick);
+                }
+            } else if (typ == 1) {  // FC(i, j): pair (i, j+1) already recorded
+                const float tot = fc_t[cf_tri(L, i, j)];
+                const int si = s[i], sj1 = s[j + 1], si1 = s[i + 1], sj = s[j];
+                const float jB = P->JB[cf_i4(si, sj1, si1, sj)];
+                const int d = j - i;
+                const float r = rng.next();
+                float c = expf(jB + T[CF_O_HP + (d <= 30 ? d : 30)] - tot);
+                if (r <= c) continue;  // hairpin
+                int pp = -1, pq = -1;
+                const int pmax = min(i + CF_CMAX, j);
+                bool done = false;
+                for (int p = i; p <= pmax && !done; ++p) {
+                    const int l1 = p - i;
+                    const int qmin = max(p + 2, p - i + j - CF_CMAX);
+                    for (int q = j; q >= qmin; --q) {
+                        if (!allow(p + 1, q)) continue;
+                        const float fc = fc_t[cf_tri(L, p + 1, q - 1)];
+                        if (!(fc > CF_HALF)) continue;
+                        const int l2 = j - q;
+                        float e;
+                        if (l1 == 0 && l2 == 0) {
+                            e = T[CF_O_BP + cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
+                        } else {
+                            float sn = 0.0f;
+                            if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
+                            else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
+                            else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
+                            e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s[p + 1], s[q])] + jB +
+                                P->JB[cf_i4(s[q], s[p + 1], s[q + 1], s[p])] + sn;
+                        }
+                        c += expf(e + fc - tot);
+                        pp = p;
+                        pq = q;
+                        if (r <= c) { done = true; break; }
+                    }
+                }
+                const float fm2 = fm2_t[cf_tri(L, i, j)];
+                if (!done && fm2 > CF_HALF) {
+                    float jA = T[CF_O_HC + cf_i2(si, sj1)];
+                    if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
+                    if (j > 0) jA = jA + T[CF_O_DR + cf_i3(si, sj1, sj)];
+                    const float wm = fm2 + jA + mp + mb;
+                    c += expf(wm - tot);
+                    {  // chosen by r, or as the last candidate when round-off left r above the sum
+                        // multiloop: choose the split k with weight FM1[i][k] + FM[k][j] over FM2[i][j]
+                        const float r2 = rng.next();
+                        float c2 = 0.0f;
+                        int kk = -1;
+                        for (int k = i + 1; k < j; ++k) {
+                            const float x = fm1_t[cf_tri(L, i, k)], y = fm_t[cf_tri(L, k, j)];
+                            if (!(x > CF_HALF && y > CF_HALF)) continue;
+                            c2 += expf(x + y - fm2);
+                            kk = k;
+                            if (r2 <= c2) break;
+                        }
+                        if (kk >= 0) {
+                            push(3, i, kk);
+                            push(2, kk, j);
+                            continue;
+                        }
+                    }
+                }
+                if (pp >= 0) {
+                    o[pp] = '(';
+                    o[pq - 1] = ')';
+                    push(1, pp + 1, pq - 1);
+                }
+            } else if (typ == 2) {  // FM(i, j): split, j unpaired, or one branch (in that order)
+                const float tot = fm_t[cf_tri(L, i, j)];
+                const float r = rng.next();
+                const float fm2 = fm2_t[cf_tri(L, i, j)];
+                const float wun = fm_t[cf_tri(L, i, j - 1)];
+                const float w1 = fm1_t[cf_tri(L, i, j)];
+                const bool h2 = fm2 > CF_HALF, hu = wun > CF_HALF, h1 = w1 > CF_HALF;
+                float c = 0.0f;
+                int opt = -1;
+                if (h2) { c += expf(fm2 - tot); if (r <= c || !(hu || h1)) opt = 0; }
+                if (opt < 0 && hu) { c += expf(wun + mu - tot); if (r <= c || !h1) opt = 1; }
+                if (opt < 0 && h1) opt = 2;
+                if (opt == 0) {
+                    const float r2 = rng.next();
+                    float c2 = 0.0f;
+                    int kk = -1;
+                    for (int k = i + 1; k < j; ++k) {
+                        const float x = fm1_t[cf_tri(L, i, k)], y = fm_t[cf_tri(L, k, j)];
+                        if (!(x > CF_HALF && y > CF_HALF)) continue;
+                        c2 += expf(x + y - fm2);
+                        kk = k;
+                        if (r2 <= c2) break;
+                    }
+                    if (kk >= 0) { push(3, i, kk); push(2, kk, j); }
+                } else if (opt == 1) {
+                    push(2, i, j - 1);
+                } else if (opt == 2) {
+                    push(3, i, j);
+                }
+            } else {  // FM1(i, j): pair (i+1, j) or i+1 unpaired
+                const float tot = fm1_t[cf_tri(L, i, j)];
+                const float r = rng.next();
+                float c = 0.0f;
+                bool paired = false;
+                if (allow(i + 1, j)) {
+                    const float fc = fc_t[cf_tri(L, i + 1, j - 1)];
+                    if (fc > CF_HALF) {
+                        const int si1 = s[i + 1], sj = s[j];
+                        float jA = T[CF_O_HC + cf_i2(sj, si1)];
+                        if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, s[j + 1])];
+                        if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, s[i])];
+                        c += expf(fc + jA + mp + T[CF_O_BP + cf_i2(si1, sj)] - tot);
+                        const float wun = fm1_t[cf_tri(L, i + 1, j)];
+                        paired = (r <= c) || !(wun > CF_HALF);
+                    }
+                }
+                if (paired) {
+                    o[i] = '(';
+                    o[j - 1] = ')';
+                    push(1, i + 1, j - 1);
+                } else if (fm1_t[cf_tri(L, i + 1, j)] > CF_HALF) {
+                    push(3, i + 1, j);
+                }
+            }
+        }
+    }
+}
+
+// ================================================================== gradient
+// Expected feature counts: every decomposition event's probability
+// exp(outside + event + inside - Z) is added to each parameter-table entry the
+// event's score uses.  The counts equal d logZ / d table (the outside pass is
+// the adjoint of the inside pass).  Accumulated per sequence in LDS, then
+// weighted into a global FP64 vector.
+__device__ __forceinline__ void cf_cnt(float* cnt, int idx, float p) { atomicAdd(&cnt[idx], p); }
+
+__global__ void __launch_bounds__(256) cf_grad_kernel(
+    const CfSeq* __restrict__ seqs, const int* __restrict__ order, const uint8_t* __restrict__ codes,
+    const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
+    const float* __restrict__ FM, const float* __restrict__ FM1, const float* __restrict__ FM2,
+    const float* __restrict__ F5, const float* __restrict__ FCo, const float* __restrict__ FMo,
+    const float* __restrict__ FM1o, const float* __restrict__ F5o, const float* __restrict__ weights,
+    double* __restrict__ grad) {
+    __shared__ float cnt[CF_NTAB];
+    const int bseq = order[blockIdx.x];
+    const CfSeq sd = seqs[bseq];
+    const int L = sd.L;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    for (int t = tid; t < CF_NTAB; t += NT) cnt[t] = 0.0f;
+    __syncthreads();
+    const uint8_t* __restrict__ s = codes + sd.coff;
+    const uint8_t* __restrict__ fo = masks ? masks + sd.coff : nullptr;
+    auto allow = [&](int a, int c) -> bool {
+        return cf_canon(s[a], s[c]) && (fo == nullptr || (fo[a] == 0 && fo[c] == 0));
+    };
+    const float* __restrict__ T = P->t;
+    const float mb = T[CF_O_MB], mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
+    const float* __restrict__ fc_t = FC + sd.toff;
+    const float* __restrict__ fm_t = FM + sd.toff;
+    const float* __restrict__ fm1_t = FM1 + sd.toff;
+    const float* __restrict__ fm2_t = FM2 + sd.toff;
+    const float* __restrict__ fco_t = FCo + sd.toff;
+    const float* __restrict__ fmo_t = FMo + sd.toff;
+    const float* __restrict__ fm1o_t = FM1o + sd.toff;
+    const float* __restrict__ f5 = F5 + sd.coff;
+    const float* __restrict__ f5o = F5o + sd.coff;
+    const float Z = f5[L];
+    for (int d = 0; d <= L; ++d)
+    for (int i = tid; i <= L - d; i += NT) {
+        const int j = i + d;
+        const int64_t c = cf_dbase(L, d) + i;
+        if (i > 0 && j < L && allow(i, j + 1)) {
+            const float fco = fco_t[c];
+            if (fco > CF_HALF) {
+                const int si = s[i], sj1 = s[j + 1], si1 = s[i + 1], sj = s[j];
+                const int jbi = cf_i4(si, sj1, si1, sj);
+                const float jB = P->JB[jbi];
+                const int hp = d <= 30 ? d : 30;
+                float pr = expf(fco + jB + T[CF_O_HP + hp] - Z);
+                cf_cnt(cnt, CF_O_HC + cf_i2(si, sj1), pr);
+                cf_cnt(cnt, CF_O_TM + jbi, pr);
+                cf_cnt(cnt, CF_O_HP + hp, pr);
+                const int pmax = min(i + CF_CMAX, j);
+                for (int p = i; p <= pmax; ++p) {
+                    const int l1 = p - i;
+                    const int qmin = max(p + 2, p - i + j - CF_CMAX);
+                    for (int q = j; q >= qmin; --q) {
+                        if (!allow(p + 1, q)) continue;
+                        const float fc = fc_t[cf_tri(L, p + 1, q - 1)];
+                        if (!(fc > CF_HALF)) continue;
+                        const int l2 = j - q;
+                        if (l1 == 0 && l2 == 0) {
+                            const float e = T[CF_O_BP + cf_i2(si1, sj)] + T[CF_O_STACK + jbi];
+                            pr = expf(fco + e + fc - Z);
+                            cf_cnt(cnt, CF_O_BP + cf_i2(si1, sj), pr);
+                            cf_cnt(cnt, CF_O_STACK + jbi, pr);
+                        } else {
+                            const int jb2 = cf_i4(s[q], s[p + 1], s[q + 1], s[p]);
+                            int sni = -1;
+                            if (l1 == 0 && l2 == 1) sni = CF_O_B01 + sj;
+                            else if (l1 == 1 && l2 == 0) sni = CF_O_B01 + si1;
+                            else if (l1 == 1 && l2 == 1) sni = CF_O_I11 + cf_i2(si1, sj);
+                            const float e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s[p + 1], s[q])] + jB +
+                                            P->JB[jb2] + (sni >= 0 ? T[sni] : 0.0f);
+                            pr = expf(fco + e + fc - Z);
+                            cf_cnt(cnt, CF_O_CS + l1 * 31 + l2, pr);
+                            cf_cnt(cnt, CF_O_BP + cf_i2(s[p + 1], s[q]), pr);
+                            cf_cnt(cnt, CF_O_HC + cf_i2(si, sj1), pr);
+                            cf_cnt(cnt, CF_O_TM + jbi, pr);
+                            cf_cnt(cnt, CF_O_HC + cf_i2(s[q], s[p + 1]), pr);
+                            cf_cnt(cnt, CF_O_TM + jb2, pr);
+                            if (sni >= 0) cf_cnt(cnt, sni, pr);
+                        }
+                    }
+                }
+                const float fm2 = fm2_t[c];
+                if (fm2 > CF_HALF) {
+                    float jA = T[CF_O_HC + cf_i2(si, sj1)];
+                    if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
+                    if (j > 0) jA = jA + T[CF_O_DR + cf_i3(si, sj1, sj)];
+                    pr = expf(fco + fm2 + jA + mp + mb - Z);
+                    cf_cnt(cnt, CF_O_HC + cf_i2(si, sj1), pr);
+                    if (i < L) cf_cnt(cnt, CF_O_DL + cf_i3(si, sj1, si1), pr);
+                    if (j > 0) cf_cnt(cnt, CF_O_DR + cf_i3(si, sj1, sj), pr);
+                    cf_cnt(cnt, CF_O_MP, pr);
+                    cf_cnt(cnt, CF_O_MB, pr);
+                }
+            }
+        }
+        if (i > 0 && i + 2 <= j && j < L) {
+            const float f1o = fm1o_t[c];
+            if (f1o > CF_HALF) {
+                if (allow(i + 1, j)) {
+                    const float fc = fc_t[cf_tri(L, i + 1, j - 1)];
+                    if (fc > CF_HALF) {
+                        const int si1 = s[i + 1], sj = s[j];
+                        float jA = T[CF_O_HC + cf_i2(sj, si1)];
+                        if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, s[j + 1])];
+                        if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, s[i])];
+                        const float pr = expf(f1o + fc + jA + mp + T[CF_O_BP + cf_i2(si1, sj)] - Z);
+                        cf_cnt(cnt, CF_O_HC + cf_i2(sj, si1), pr);
+                        if (j < L) cf_cnt(cnt, CF_O_DL + cf_i3(sj, si1, s[j + 1]), pr);
+                        if (i > 0) cf_cnt(cnt, CF_O_DR + cf_i3(sj, si1, s[i]), pr);
+                        cf_cnt(cnt, CF_O_MP, pr);
+                        cf_cnt(cnt, CF_O_BP + cf_i2(si1, sj), pr);
+                    }
+                }
+                const float f1n = fm1_t[cf_tri(L, i + 1, j)];
+                if (f1n > CF_HALF) cf_cnt(cnt, CF_O_MU, expf(f1o + f1n + mu - Z));
+            }
+            const float fmo = fmo_t[c];
+            if (fmo > CF_HALF) {
+                const float fml = fm_t[cf_tri(L, i, j - 1)];
+                if (fml > CF_HALF) cf_cnt(cnt, CF_O_MU, expf(fmo + fml + mu - Z));
+            }
+        }
+    }
+    // exterior loop events
+    for (int j = 1 + tid; j <= L; j += NT) {
+        cf_cnt(cnt, CF_O_EU, expf(f5o[j] + f5[j - 1] + eu - Z));
+        const int sj = s[j];
+        for (int k = 0; k < j; ++k) {
+            if (!allow(k + 1, j)) continue;
+            const float fc = fc_t[cf_tri(L, k + 1, j - 1)];
+            if (!(fc > CF_HALF) || !(f5[k] > CF_HALF)) continue;
+            const int sk1 = s[k + 1];
+            float jA = T[CF_O_HC + cf_i2(sj, sk1)];
+            if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s[j + 1])];
+            if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s[k])];
+            const float pr = expf(f5o[j] + f5[k] + fc + ep + T[CF_O_BP + cf_i2(sk1, sj)] + jA - Z);
+            cf_cnt(cnt, CF_O_EP, pr);
+            cf_cnt(cnt, CF_O_BP + cf_i2(sk1, sj), pr);
+            cf_cnt(cnt, CF_O_HC + cf_i2(sj, sk1), pr);
+            if (j < L) cf_cnt(cnt, CF_O_DL + cf_i3(sj, sk1, s[j + 1]), pr);
+            if (k > 0) cf_cnt(cnt, CF_O_DR + cf_i3(sj, sk1, s[k]), pr);
+        }
+    }
+    __syncthreads();
+    const double w = weights ? (double)weights[bseq] : 1.0;
+    for (int t = tid; t < CF_NTAB; t += NT)
+        if (cnt[t] != 0.0f) atomicAdd(&grad[t], w * (double)cnt[t]);
+}
+
+// ------------------------------------------------------------ explicit inits
+template __global__ void cf_inside_kernel<true>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
+                                                const CfParams<float>*, float*, float*, float*, float*, float*, int);
+template __global__ void cf_inside_kernel<false>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
+                                                 const CfParams<float>*, float*, float*, float*, float*, float*, int);
+template __global__ void cf_outside_kernel<true>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
+                                                 const CfParams<float>*, const float*, const float*, const float*,
+                                                 const float*, const float*, float*, float*, float*, float*, float*,
+                                                 int);
+template __global__ void cf_outside_kernel<false>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
+                                                  const CfParams<float>*, const float*, const float*, const float*,
+                                                  const float*, const float*, float*, float*, float*, float*, float*,
+                                                  int);
+template __global__ void cf_viterbi_kernel<true>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
+                                                 const CfParams<double>*, double*, double*, double*, double*, uint8_t*,
+                                                 const int64_t*, int);
+template __global__ void cf_viterbi_kernel<false>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
+                                                  const CfParams<double>*, double*, double*, double*, double*,
+                                                  uint8_t*, const int64_t*, int);

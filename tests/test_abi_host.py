@@ -1,0 +1,167 @@
+"""CPU-side checks: the C ABI library loads and exports every entry point
+declared in include/*.h; host logic (packing, masks, keys, parameter map,
+workload generators, work counts) behaves."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(cf_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    from contrafold_gpu_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from contrafold_gpu_amd import build
+        build.build(force=True)
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    decl = declared_symbols()
+    assert len(decl) >= 30
+    missing = [n for n in sorted(decl) if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {name for name, _, _ in _lib.SIGNATURES}
+    assert decl == bound, (decl ^ bound)
+
+
+def test_abi_constants_without_gpu():
+    from contrafold_gpu_amd import _lib, params
+    lib = _lib.load_library()
+    assert lib.cf_table_size() == params.N_TABLE == 2577
+    assert lib.cf_version() >= 1
+    assert lib.cf_kernel_count() == 7
+    assert lib.cf_kernel_name(0) == b"inside"
+    # admission bytes grow like L^2 and with the stages requested
+    b200 = lib.cf_fold_bytes(200, _lib.CF_LOGZ, 1, 0)
+    b400 = lib.cf_fold_bytes(400, _lib.CF_LOGZ, 1, 0)
+    assert 3.5 < b400 / b200 < 4.5
+    assert lib.cf_fold_bytes(200, _lib.CF_MEA, 1, 0) > lib.cf_fold_bytes(200, _lib.CF_BPP, 1, 0) > b200
+    assert lib.cf_fold_bytes(-1, _lib.CF_LOGZ, 1, 0) == -1
+
+
+def test_null_and_bad_arguments_return_status():
+    import ctypes
+    from contrafold_gpu_amd import _lib
+    lib = _lib.load_library()
+    assert lib.cf_create(0, None) == _lib.CF_EINVAL
+    assert lib.cf_set_params(None, None) == _lib.CF_EINVAL
+    assert lib.cf_batch_run(None, 1, None, 0, 0, 0, None) == _lib.CF_EINVAL
+    assert lib.cf_batch_logz(None, None) == _lib.CF_EINVAL
+    assert lib.cf_last_error(None) == b"null context"
+    lib.cf_destroy(None)
+    lib.cf_batch_destroy(None)
+    lib.cf_server_destroy(None)
+
+
+def test_status_mapping():
+    from contrafold_gpu_amd import _lib
+    with pytest.raises(ValueError):
+        _lib.check(_lib.CF_EINVAL)
+    with pytest.raises(MemoryError):
+        _lib.check(_lib.CF_ENOMEM)
+    with pytest.raises(RuntimeError):
+        _lib.check(_lib.CF_EDEVICE)
+    assert _lib.check(3) == 3
+
+
+def test_encode_batch_masks_and_errors():
+    from contrafold_gpu_amd.engine import encode_batch, sequence_keys
+    buf, offs, mbuf, lens = encode_batch(["ACGU", "", "gg"], [None, None, [1, 0, 5]])
+    assert buf == b"ACGUgg" and list(offs) == [0, 4, 4, 6] and list(lens) == [4, 0, 2]
+    assert list(mbuf) == [0, 0, 0, 0, 1, 0]          # truncated to L, nonzero -> 1
+    assert encode_batch(["AC"], [None])[2] is None
+    with pytest.raises(ValueError):
+        encode_batch([], None)
+    with pytest.raises(ValueError):
+        encode_batch(["ACGU"], [[0, 1]])
+    with pytest.raises(ValueError):
+        encode_batch(["ACGU", "A"], [None])
+    with pytest.raises(TypeError):
+        encode_batch("ACGU")
+    # non-ASCII characters become one placeholder byte each (-> N)
+    b2, o2, _, l2 = encode_batch(["AéU"])
+    assert l2[0] == 3
+    k1 = sequence_keys(*encode_batch(["ACGU", "GGG", "ACGU"])[:3])
+    k2 = sequence_keys(*encode_batch(["GGG", "ACGU"])[:3])
+    assert k1[0] == k2[1] and k1[2] != k1[0]          # content keys, repeats distinct
+
+
+def test_param_map_linear_and_gradient_transpose():
+    from contrafold_gpu_amd import params
+    names, raw, pm = params.load_raw()
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=raw.size)
+    y = rng.normal(size=params.N_TABLE)
+    J = pm.dense()
+    assert np.allclose(pm.tables(x), J @ x, atol=1e-12)
+    assert np.allclose(pm.raw_grad(y), J.T @ y, atol=1e-12)
+    assert pm.tables(raw).tobytes() == params.flatten(params.load()).tobytes()
+
+
+def test_loader_matches_oracle_restatement(oracle):
+    from contrafold_gpu_amd import params
+    P = params.load()
+    Q = oracle.load_tables()
+    for k in Q:
+        assert np.array_equal(np.asarray(P[k]), np.asarray(Q[k])), k
+
+
+def test_loader_table_identities(P):
+    """Symmetries and running-sum identities of cpu.load (cpu.py:51-130)."""
+    from contrafold_gpu_amd import params
+    names, raw = params.read_raw()
+    rv = dict(zip(names, raw))
+    assert np.array_equal(P["bp"], P["bp"].T) and np.array_equal(P["i11"], P["i11"].T)
+    assert not P["bp"][4].any() and not P["stack"][4].any()          # N row stays 0
+    d = np.diff(P["hp_cum"])
+    assert np.allclose(d, [rv[f"hairpin_length_at_least_{k}"] for k in range(1, 31)])
+    assert P["hp_cum"][0] == rv["hairpin_length_at_least_0"]
+    assert P["cs"][0, 5] == P["cs"][5, 0] == sum(rv[f"bulge_length_at_least_{k}"] for k in range(1, 6))
+    # stacking falls back to the reversed key
+    for w in ("CGCG", "AUGC", "GCUA"):
+        a, b, c, dd = (params.IDX[ch] for ch in w)
+        want = rv.get("helix_stacking_" + w, rv.get("helix_stacking_" + w[::-1], 0.0))
+        assert P["stack"][a, b, c, dd] == want
+    for s in ("mb", "mu", "mp", "eu", "ep"):
+        assert isinstance(P[s], float)
+
+
+def test_workload_generators():
+    from contrafold_gpu_amd import workloads
+    a = workloads.uniform(5, 30, seed=0)
+    assert a == workloads.uniform(5, 30, seed=0) and all(len(s) == 30 for s in a)
+    seqs, masks = workloads.gc_masked(50, 100, seed=2)
+    frac = np.mean([m.mean() for m in masks])
+    assert 0.07 < frac < 0.13
+    u = workloads.uneven(200, seed=3)
+    L = np.array([len(s) for s in u])
+    assert L.min() >= 30 and L.max() <= 500
+    nfrac = np.mean([s.count("N") / len(s) for s in u])
+    assert 0.01 < nfrac < 0.03
+    tr = workloads.serving_trace(2000, 100.0, seed=5)
+    assert np.all(np.diff(tr["arrival_s"]) >= 0)
+    assert tr["lengths"].min() >= 20 and tr["lengths"].max() <= 1000
+    assert 140 <= np.median(tr["lengths"]) <= 160
+    kinds = np.array(tr["kinds"])
+    assert abs((kinds == "mea").mean() - 0.7) < 0.04
+
+
+def test_work_counts_closed_forms():
+    from contrafold_gpu_amd import workcount
+    c = workcount.counts("A" * 50)        # nothing pairs: no FC cells, no windows
+    assert c["posterior"] == 0 and c["exterior"] == 100
+    c = workcount.counts("GGGGAAAACCCC")
+    assert c["cells"] == 13 * 14 // 2 and c["inside"] > 0

@@ -1,0 +1,244 @@
+"""GPU feature tests: sampling statistics and constraints, gradients against
+FP64 finite differences, graph replay, launch-shape invariance, serving."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rand_seq
+
+pytestmark = pytest.mark.gpu
+
+
+def pairs_of(db):
+    st, out = [], []
+    for k, ch in enumerate(db):
+        if ch == "(":
+            st.append(k)
+        elif ch == ")":
+            out.append((st.pop(), k))
+    assert not st
+    return out
+
+
+CANON = {("A", "U"), ("U", "A"), ("C", "G"), ("G", "C"), ("G", "U"), ("U", "G")}
+
+
+def test_sample_frequencies_match_bpp(P):
+    """Reference test 3 (tests/test_validation.py:44-64) against our own bpp:
+    20,000 samples of GCGCGCAAAAGCGCGCAAAAGCGC, seed 1."""
+    import contrafold_gpu_amd as cf
+    s = "GCGCGCAAAAGCGCGCAAAAGCGC"
+    L = len(s)
+    n = 20000
+    samples = cf.sample_batch([s], P, n, seed=1)[0]
+    assert len(samples) == n
+    freq = np.zeros((L, L))
+    for db in samples:
+        for a, b in pairs_of(db):
+            freq[a, b] += 1
+    freq /= n
+    post = cf.bpp(s, P)
+    assert np.abs(freq - post).max() < 0.02
+    sigma = np.sqrt(post * (1 - post) / n)
+    assert (np.abs(freq - post) <= 5 * sigma + 2e-3).all()
+
+
+def test_sample_masks_and_validity(P):
+    """Reference test 4 (tests/test_validation.py:66-76): forced bases never
+    pair; every sample is a nested structure of canonical pairs."""
+    import contrafold_gpu_amd as cf
+    rng = np.random.default_rng(0)
+    seqs, masks = [], []
+    for t in range(12):
+        L = int(rng.integers(10, 90))
+        seqs.append(rand_seq(rng, L))
+        masks.append((rng.random(L) < 0.3).astype(np.int8))
+    res = cf.sample_batch(seqs, P, 300, forced_list=masks, seed=2)
+    for s, m, smp in zip(seqs, masks, res):
+        for db in smp:
+            assert len(db) == len(s)
+            for a, b in pairs_of(db):
+                assert not m[a] and not m[b]
+                assert (s[a], s[b]) in CANON
+
+
+def test_sampling_batch_invariant(P):
+    import contrafold_gpu_amd as cf
+    rng = np.random.default_rng(4)
+    s = rand_seq(rng, 80)
+    others = [rand_seq(rng, int(rng.integers(20, 150))) for _ in range(7)]
+    alone = cf.sample_batch([s], P, 50, seed=7)[0]
+    mixed = cf.sample_batch(others[:3] + [s] + others[3:], P, 50, seed=7)[3]
+    assert alone == mixed
+    assert cf.sample_batch([s], P, 50, seed=8)[0] != alone
+
+
+def test_fold_tasks_gpu_order(P):
+    import contrafold_gpu_amd as cf
+    rng = np.random.default_rng(1)
+    seqs = [rand_seq(rng, int(rng.integers(5, 60))) for _ in range(30)]
+    out = cf.fold_tasks_gpu(seqs, None, P, chunk=7, seed=3)
+    assert [len(x) for x in out] == [len(s) for s in seqs]
+
+
+def test_gradient_vs_fp64_finite_differences(P, oracle):
+    """d mean logZ / d raw weight from the outside-pass adjoint against central
+    differences (step 1e-4) of the oracle's FP64 inside pass."""
+    import contrafold_gpu_amd as cf
+    from contrafold_gpu_amd import params
+    names, raw, pmap = params.load_raw()
+    rng = np.random.default_rng(4)
+    seqs = [rand_seq(rng, 40) for _ in range(6)]
+    w = np.full(len(seqs), 1.0 / len(seqs), np.float32)
+    _z, tg = cf.logZ_grad(seqs, P, weights=w)
+    g = pmap.raw_grad(tg)
+    # canonical-reachable params have nonzero gradient somewhere; pick both kinds
+    idx = list(rng.choice(len(raw), 24, replace=False))
+    idx += [names.index(k) for k in ("multi_base", "multi_unpaired", "external_unpaired", "base_pair_CG",
+                                      "hairpin_length_at_least_3", "helix_stacking_CGCG")]
+    h = 1e-4
+
+    def mean_logz(r):
+        Q = params.unflatten(pmap.tables(r))
+        return np.mean([oracle.inside64(s, Q) for s in seqs])
+    for k in idx:
+        e = np.zeros_like(raw)
+        e[k] = h
+        fd = (mean_logz(raw + e) - mean_logz(raw - e)) / (2 * h)
+        assert abs(g[k] - fd) <= 2e-3 * max(1.0, abs(fd)), (names[k], g[k], fd)
+
+
+def test_gradient_closed_form_unpairable(P):
+    """No pair possible: logZ = L * eu, so d logZ / d eu = L and nothing else moves."""
+    import contrafold_gpu_amd as cf
+    from contrafold_gpu_amd import params
+    _z, tg = cf.logZ_grad(["AAAAAAAAAAAA", "NNNNNNN"], P)
+    assert abs(tg[params.OFFSETS["eu"]] - 19.0) < 1e-4
+    tg[params.OFFSETS["eu"]] = 0
+    assert np.abs(tg).max() < 1e-6
+
+
+def test_torch_autograd(P):
+    torch = pytest.importorskip("torch")
+    import contrafold_gpu_amd as cf
+    from contrafold_gpu_amd import autograd, params
+    rng = np.random.default_rng(9)
+    seqs = [rand_seq(rng, 50) for _ in range(4)]
+    theta = autograd.raw_theta()
+    z = autograd.logz(theta, seqs)
+    z.mean().backward()
+    _z, tg = cf.logZ_grad(seqs, P, weights=np.full(4, 0.25, np.float32))
+    want = params.param_map().raw_grad(tg)
+    assert np.allclose(theta.grad.numpy(), want, rtol=1e-5, atol=1e-7)
+    assert np.allclose(z.detach().numpy(), cf.logZ_batch(seqs, P))
+
+
+def test_graph_replay_matches_eager(engine, P):
+    from contrafold_gpu_amd import CF_BPP
+    from contrafold_gpu_amd._lib import CF_GRAPH
+    rng = np.random.default_rng(12)
+    seqs = [rand_seq(rng, int(rng.integers(30, 130))) for _ in range(40)]
+    with engine.batch(seqs) as b:
+        b.run(CF_BPP, recompute=True)
+        z0 = b.logz().copy()
+        p0 = b.bpp(3).copy()
+        for _ in range(3):
+            b.run(CF_BPP | CF_GRAPH)
+            assert (b.logz().view(np.uint32) == z0.view(np.uint32)).all()
+            assert (b.bpp(3).view(np.uint32) == p0.view(np.uint32)).all()
+            b.run(CF_BPP, recompute=True)
+
+
+def test_launch_shape_invariance(P):
+    """Results are bit-identical across bucket edges and workgroup sizes (a
+    cross-lane race would show up as a mismatch)."""
+    from contrafold_gpu_amd import CF_BPP, Engine
+    rng = np.random.default_rng(13)
+    seqs = [rand_seq(rng, int(rng.integers(10, 260))) for _ in range(30)]
+    ref = None
+    for edges, threads in (([1 << 30], [64]), ([1 << 30], [256]), ([100, 1 << 30], [128, 192]),
+                           ([64, 128, 256, 1 << 30], [64, 128, 256, 256])):
+        eng = Engine()
+        eng.set_params(P)
+        eng.set_tuning(edges, threads)
+        with eng.batch(seqs) as b:
+            b.run(CF_BPP)
+            got = (b.logz().copy(), [b.bpp(k) for k in range(0, 30, 7)])
+        eng.close()
+        if ref is None:
+            ref = got
+        else:
+            assert (got[0].view(np.uint32) == ref[0].view(np.uint32)).all()
+            for x, y in zip(got[1], ref[1]):
+                assert (x.view(np.uint32) == y.view(np.uint32)).all()
+
+
+def test_repeat_determinism(engine, P):
+    from contrafold_gpu_amd import CF_MEA
+    rng = np.random.default_rng(14)
+    seqs = [rand_seq(rng, 200) for _ in range(64)]
+    with engine.batch(seqs) as b:
+        b.run(CF_MEA, gammas=[6.0], recompute=True)
+        a = (b.logz().copy(), b.mea(0), b.bpp(5).copy())
+        b.run(CF_MEA, gammas=[6.0], recompute=True)
+        assert (b.logz().view(np.uint32) == a[0].view(np.uint32)).all()
+        assert b.mea(0) == a[1]
+        assert (b.bpp(5).view(np.uint32) == a[2].view(np.uint32)).all()
+
+
+def test_server_mix_equals_alone(engine, P):
+    """Each served result equals the same request served alone, and MEA/bpp/
+    logZ equal the plain batch API bit for bit."""
+    import contrafold_gpu_amd as cf
+    from contrafold_gpu_amd.serving import Server
+    rng = np.random.default_rng(15)
+    reqs = []
+    for t in range(40):
+        L = int(rng.integers(5, 220))
+        s = rand_seq(rng, L)
+        m = (rng.random(L) < 0.15).astype(np.uint8) if t % 3 == 0 else None
+        kind = ("mea", "mea", "sample", "bpp", "logz")[t % 5]
+        reqs.append((s, m, kind))
+    srv = Server(engine, n_streams=3, max_batch=16)
+    ids = [srv.submit(s, kind=k, mask=m, n_samples=10, seed=11) for s, m, k in reqs]
+    mixed = srv.run_all()
+    srv.close()
+    for rid, (s, m, kind) in zip(ids, reqs):
+        r = mixed[rid]
+        z = cf.logZ_batch([s], P, forced_list=[m])[0]
+        assert np.float32(r["logz"]) == np.float32(z)
+        if kind == "mea":
+            assert r["structure"] == cf.mea(s, P, 6.0, m)
+        elif kind == "bpp":
+            assert (r["bpp"] == cf.bpp(s, P, m)).all()
+    # alone
+    for rid, (s, m, kind) in list(zip(ids, reqs))[::4]:
+        srv = Server(engine, n_streams=1)
+        rid2 = srv.submit(s, kind=kind, mask=m, n_samples=10, seed=11)
+        r2 = srv.run_all()[rid2]
+        srv.close()
+        for key in ("structure", "samples", "logz"):
+            if key in r2:
+                assert r2[key] == mixed[rid][key]
+
+
+def test_server_admission_budget(engine, P):
+    """A request larger than the HBM budget fails cleanly; the rest complete."""
+    from contrafold_gpu_amd.serving import Server
+    from contrafold_gpu_amd import fold_bytes, CF_MEA
+    small = fold_bytes(60, CF_MEA)
+    srv = Server(engine, hbm_budget=3 * small, n_streams=2, max_batch=8)
+    rng = np.random.default_rng(16)
+    ids = [srv.submit(rand_seq(rng, 60)) for _ in range(7)]
+    big = srv.submit(rand_seq(rng, 400))
+    done = {}
+    while len(done) < 8:
+        for rid in srv.poll():
+            try:
+                done[rid] = srv.result(rid)
+            except MemoryError:
+                done[rid] = None
+    srv.close()
+    assert done[big] is None
+    assert all(done[i] is not None for i in ids)

@@ -1,0 +1,197 @@
+"""GPU parity: the gfx950 kernels against the oracle, bit for bit.
+
+Every FP32 DP table (inside FC/FM/FM1/F5, outside FCo/FMo/FM1o/F5o, the
+posterior POST) must equal the oracle's restatement of the reference's float32
+engine (gpu_contrafold/cpu.py:583-820) to the bit on every cell of its domain;
+non-finite cells must be non-finite in both.  MEA (every gamma) and Viterbi
+dot-brackets must be identical strings.  FP64 logZ must match the oracle's
+exact inside pass to 1e-9.
+"""
+import numpy as np
+import pytest
+
+from conftest import rand_seq
+
+pytestmark = pytest.mark.gpu
+
+HALF = np.float32(-5e29)
+CANON = {(0, 3), (3, 0), (1, 2), (2, 1), (2, 3), (3, 2)}
+
+
+def cases(seed=11):
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(36):
+        L = int(rng.integers(1, 72))
+        g = rng.beta(8, 8)
+        seq = rand_seq(rng, L, p=[(1 - g) / 2, g / 2, g / 2, (1 - g) / 2])
+        if t % 5 == 4:
+            seq = "".join(c if rng.random() > 0.05 else "N" for c in seq)
+        mask = (rng.random(L) < 0.1).astype(np.int8) if t % 3 == 0 else None
+        out.append((seq, mask))
+    out += [("", None), ("A", None), ("AU", None), ("GGGGAAAACCCC", None), ("N" * 9, None),
+            ("gcgcaaagcgcuuuugcgcaaagcgc", None), ("GCGCGCAAAAGCGCGCAAAAGCGC", np.ones(24, np.int8)),
+            ("GGGAAACCCAGGGAAACCCUGGGAAACCC", None), (rand_seq(np.random.default_rng(5), 160), None)]
+    return out
+
+
+CASES = cases()
+
+
+def encode(seq):
+    m = {"A": 0, "C": 1, "G": 2, "U": 3, "T": 3}
+    return [4] + [m.get(c, 4) for c in seq.upper()] + [4]
+
+
+def domains(seq, mask):
+    s = encode(seq)
+    L = len(seq)
+    fo = [0] + [int(x != 0) for x in (mask if mask is not None else np.zeros(L))] + [0]
+
+    def allow(a, b):
+        return (s[a], s[b]) in CANON and fo[a] == 0 and fo[b] == 0
+    fc = [(i, j) for i in range(1, L + 1) for j in range(i, L) if allow(i, j + 1)]
+    fm = [(i, j) for i in range(1, L + 1) for j in range(i + 2, L)]
+    tri = [(i, j) for i in range(L + 1) for j in range(i, L + 1)]
+    return fc, fm, tri
+
+
+def same_bits(g, c):
+    g = np.asarray(g, np.float32)
+    c = np.asarray(c, np.float32)
+    both_inf = (g <= HALF) & (c <= HALF)
+    return both_inf | (g.view(np.uint32) == c.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def folded(engine, P, oracle):
+    from contrafold_gpu_amd import CF_BPP
+    seqs = [c[0] for c in CASES]
+    masks = [c[1] for c in CASES]
+    b = engine.batch(seqs, masks)
+    b.run(CF_BPP)
+    refs = [oracle.posterior32(s, P, m, tables=True) for s, m in CASES]
+    yield b, refs
+    b.close()
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_inside_tables_bitexact(folded, k):
+    b, refs = folded
+    seq, mask = CASES[k]
+    L = len(seq)
+    z, _POST, T = refs[k]
+    fc, fm, tri = domains(seq, mask)
+    for name, cells in (("FC", fc), ("FM", fm), ("FM1", fm)):
+        g = b.table(k, name)
+        if cells:
+            ii, jj = np.array(cells).T
+            ok = same_bits(g[ii, jj], T[name][ii, jj])
+            assert ok.all(), f"{name} differs at {np.array(cells)[~ok][:5]}"
+    # cells outside the FC domain stay non-finite
+    g = b.table(k, "FC")
+    ii, jj = np.array(tri).T
+    inside = set(fc)
+    off = np.array([(i, j) not in inside for i, j in tri])
+    assert (g[ii[off], jj[off]] <= HALF).all()
+    f5 = b.table(k, "F5")
+    assert same_bits(f5[:L + 1], T["F5"][:L + 1]).all()
+    assert b.logz()[k].view(np.uint32) == np.float32(z).view(np.uint32)
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_outside_tables_bitexact(folded, k):
+    b, refs = folded
+    seq, mask = CASES[k]
+    L = len(seq)
+    _z, _POST, T = refs[k]
+    fc, fm, _ = domains(seq, mask)
+    for name, cells in (("FCo", fc), ("FMo", fm), ("FM1o", fm)):
+        if not cells:
+            continue
+        g = b.table(k, name)
+        ii, jj = np.array(cells).T
+        ok = same_bits(g[ii, jj], T[name][ii, jj])
+        assert ok.all(), f"{name} differs at {np.array(cells)[~ok][:5]}"
+    f5o = b.table(k, "F5o")
+    assert same_bits(f5o[:L + 1], T["F5o"][:L + 1]).all()
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_posterior_bitexact(folded, k):
+    b, refs = folded
+    seq, _mask = CASES[k]
+    L = len(seq)
+    _z, POST, _T = refs[k]
+    got = b.bpp(k)
+    want = POST[1:L + 1, 1:L + 1]
+    assert got.shape == (L, L)
+    assert (got.view(np.uint32) == want.view(np.uint32)).all()
+
+
+def test_mea_all_gammas_bitexact(engine, P, oracle):
+    from contrafold_gpu_amd import CF_MEA
+    gammas = [0.5, 1.0, 2.0, 4.0, 6.0, 8.0, 16.0]
+    seqs = [c[0] for c in CASES]
+    masks = [c[1] for c in CASES]
+    with engine.batch(seqs, masks) as b:
+        b.run(CF_MEA, gammas=gammas)
+        for g_i, g in enumerate(gammas):
+            got = b.mea(g_i)
+            for k, (s, m) in enumerate(CASES):
+                assert got[k] == oracle.mea(s, P, g, m), (k, g)
+
+
+def test_viterbi_bitexact(engine, P, oracle):
+    from contrafold_gpu_amd import CF_MFE
+    seqs = [c[0] for c in CASES]
+    masks = [c[1] for c in CASES]
+    with engine.batch(seqs, masks) as b:
+        b.run(CF_MFE)
+        got, scores = b.mfe()
+    for k, (s, m) in enumerate(CASES):
+        pair, sc = oracle.viterbi64(s, P, m)
+        assert got[k] == oracle.dot_bracket(pair, len(s)), k
+        assert scores[k] == sc, k
+
+
+def test_logz64_matches_oracle(engine, P, oracle):
+    from contrafold_gpu_amd import CF_LOGZ64
+    seqs = [c[0] for c in CASES]
+    masks = [c[1] for c in CASES]
+    with engine.batch(seqs, masks) as b:
+        b.run(CF_LOGZ64)
+        z = b.logz64()
+    for k, (s, m) in enumerate(CASES):
+        assert abs(z[k] - oracle.inside64(s, P, m)) < 1e-9, k
+
+
+def test_dropin_api_known_answers(P):
+    import contrafold_gpu_amd as cf
+    assert cf.mfe("GGGGAAAACCCC", P) == "((((....))))"   # README.md:101
+    assert cf.mea("GGGGAAAACCCC", P, gamma=6) == "((((....))))"  # README.md:104
+    z = cf.logZ_batch(["GGGGAAAACCCC", "GCGCGCAAGCGCGC"], P)
+    assert len(z) == 2 and all(np.isfinite(z))
+    assert abs(cf.cpu_logZ("GGGGAAAACCCC", P) - z[0]) < 5e-4
+    pr = cf.bpp("GGGGAAAACCCC", P)
+    assert pr.shape == (12, 12) and pr.dtype == np.float64
+    assert cf.logZ_batch([""], P) == [0.0]
+    assert cf.mea("", P) == ""
+    with pytest.raises(ValueError):
+        cf.logZ_batch([], P)
+    with pytest.raises(ValueError):
+        cf.logZ_batch(["GGGAAACCC"], P, forced_list=[[0, 1]])
+
+
+def test_long_sequence_parity(engine, P, oracle):
+    """One 420-nt sequence (the ring-buffer path with many diagonals)."""
+    from contrafold_gpu_amd import CF_MEA
+    rng = np.random.default_rng(99)
+    s = rand_seq(rng, 420)
+    with engine.batch([s]) as b:
+        b.run(CF_MEA, gammas=[6.0])
+        z32, POST = oracle.posterior32(s, P)
+        assert b.logz()[0].view(np.uint32) == np.float32(z32).view(np.uint32)
+        got = b.bpp(0)
+        assert (got.view(np.uint32) == POST[1:421, 1:421].view(np.uint32)).all()
+        assert b.mea(0)[0] == oracle.mea(s, P, 6.0)
