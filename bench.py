@@ -40,7 +40,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seqs, masks, P, seconds_target=15.0):
+def cpu_baseline(seqs, masks, P, n_samples, seconds_target=15.0):
     """Oracle on all host cores over a bounded sample of the workload."""
     from oracle import oracle as O
     O.build()
@@ -48,17 +48,18 @@ def cpu_baseline(seqs, masks, P, seconds_target=15.0):
     # calibrate on one fold per core, then size the sample to ~seconds_target
     probe = seqs[:cores]
     t = time.perf_counter()
-    O.run_batch("bpp32", probe, P, masks=masks[:cores], threads=cores)
+    O.run_batch("bpp_sample32", probe, P, masks=masks[:cores], threads=cores, n_samples=n_samples)
     dt = max(time.perf_counter() - t, 1e-3)
     per_s = len(probe) / dt
     n = int(min(len(seqs), max(cores, per_s * seconds_target)))
     t = time.perf_counter()
-    O.run_batch("bpp32", seqs[:n], P, masks=masks[:n], threads=cores)
+    O.run_batch("bpp_sample32", seqs[:n], P, masks=masks[:n], threads=cores, n_samples=n_samples)
     dt = time.perf_counter() - t
     res = {"value": round(n / dt, 2), "unit": "folds/s", "cores": cores, "kind": "port",
-           "sample": f"{n} of the 200-nt sequences, FP32 inside+outside+posterior (logZ + bpp) of the oracle "
-                     f"(C restatement of gpu_contrafold/cpu.py:583-820, -O3, no FMA); sampling is not part "
-                     f"of the CPU fold", "seconds": round(dt, 2)}
+           "sample": f"first {n} of the same 200-nt sequences and masks: FP32 inside + outside + posterior "
+                     f"(logZ + bpp) and {n_samples} Boltzmann samples per sequence, by the oracle (C restatement "
+                     f"of gpu_contrafold/cpu.py:583-820 and gpu.py:196-477, -O3, no FMA), one fold per "
+                     f"task on a pthread pool of {cores} threads", "seconds": round(dt, 2)}
     # MEA (posterior + decode, gamma 6), the headline CPU comparison
     n2 = max(cores, n // 2)
     t = time.perf_counter()
@@ -68,8 +69,104 @@ def cpu_baseline(seqs, masks, P, seconds_target=15.0):
     return res
 
 
+def run_config(name, args):
+    """Secondary configurations (parity-scale cases with their own measurements)."""
+    import contrafold_gpu_amd as cf
+    from contrafold_gpu_amd import workloads, autotune
+    from contrafold_gpu_amd._lib import CF_GRAPH
+    P = cf.load()
+    eng = cf.default_engine()
+    eng.set_params(P)
+    out = {"config": name}
+    if name == "c1":
+        # 1,000 x 128 nt uniform: FP32 logZ + MEA (gamma 6) + Viterbi; oracle compares on the CPU
+        seqs = workloads.uniform(1000, 128, seed=0)
+        with eng.batch(seqs) as b:
+            b.run(cf.CF_MEA | cf.CF_MFE, gammas=[6.0])
+            t = time.perf_counter()
+            for _ in range(3):
+                b.run(cf.CF_MEA | cf.CF_MFE, gammas=[6.0], recompute=True)
+            dt = (time.perf_counter() - t) / 3
+            z = b.logz()
+            mea = b.mea(0)
+            vit, _ = b.mfe()
+        from oracle import oracle as O
+        O.build()
+        k = min(args.check, len(seqs))
+        z64, _ = O.run_batch("logz64", seqs[:k], P)
+        zz, pairs = O.run_batch("mea32", seqs[:k], P, gamma=6.0)
+        _v, vp = O.run_batch("viterbi64", seqs[:k], P)
+        out.update({"folds_per_s": round(len(seqs) / dt, 1),
+                    "logz_max_abs_err_vs_fp32_oracle": float(np.abs(z[:k] - zz.astype(np.float32)).max()),
+                    "logz_max_abs_err_vs_fp64": float(np.abs(z[:k].astype(np.float64) - z64).max()),
+                    "mea_identical": int(sum(mea[i] == O.dot_bracket(pairs[i], 128) for i in range(k))),
+                    "viterbi_identical": int(sum(vit[i] == O.dot_bracket(vp[i], 128) for i in range(k))),
+                    "checked": k})
+    elif name == "c3":
+        # 20,000 x U[30,500] nt with 2% N: MEA at 3 gammas + Viterbi, length buckets, hipGraph, autotuned
+        seqs = workloads.uneven(args.n if args.n != 10000 else 20000, seed=3)
+        flags = cf.CF_MEA | cf.CF_MFE
+        tuned = autotune.tune(eng, seqs, flags=flags, gammas=(0.5, 6.0, 16.0), sample=1500, repeats=1, log=log)
+        with eng.batch(seqs) as b:
+            b.run(flags | CF_GRAPH, gammas=[0.5, 6.0, 16.0])
+            ms = []
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                b.run(flags | CF_GRAPH, gammas=[0.5, 6.0, 16.0])
+                ms.append(b.last_ms())
+            dt = (time.perf_counter() - t) / args.steps
+        out.update({"folds_per_s": round(len(seqs) / dt, 1), "device_ms": round(float(np.mean(ms)), 2),
+                    "tuning": {"edges": tuned["edges"], "threads": tuned["threads"]}, "graph": True})
+        eng.set_tuning(*autotune.DEFAULT)
+    elif name == "c4":
+        # 512 x 150 nt: gradient of mean logZ w.r.t. the 708 raw weights vs FP64 central differences
+        from contrafold_gpu_amd import params
+        names, raw, pmap = params.load_raw()
+        seqs = workloads.uniform(512, 150, seed=4)
+        w = np.full(len(seqs), 1.0 / len(seqs), np.float32)
+        with eng.batch(seqs) as b:
+            b.run(cf.CF_GRAD, weights=w)
+            t = time.perf_counter()
+            for _ in range(3):
+                b.run(cf.CF_GRAD, weights=w, recompute=True)
+            dt = (time.perf_counter() - t) / 3
+            g = pmap.raw_grad(b.grad())
+        from oracle import oracle as O
+        O.build()
+        rng = np.random.default_rng(4)
+        idx = rng.choice(len(raw), 32, replace=False)
+        sub = seqs[:args.check]
+        h = 1e-4
+        rows = []
+        for kk in idx:
+            e = np.zeros_like(raw)
+            e[kk] = h
+            fp = O.run_batch("logz64", sub, params.unflatten(pmap.tables(raw + e)))[0].mean()
+            fm = O.run_batch("logz64", sub, params.unflatten(pmap.tables(raw - e)))[0].mean()
+            rows.append((names[kk], float((fp - fm) / (2 * h))))
+        # the GPU gradient over the same subset for the comparison
+        with eng.batch(sub) as b:
+            b.run(cf.CF_GRAD, weights=np.full(len(sub), 1.0 / len(sub), np.float32))
+            gs = pmap.raw_grad(b.grad())
+        err = [abs(gs[names.index(n)] - fd) for n, fd in rows]
+        out.update({"grad_seconds": round(dt, 4), "seqs": len(seqs),
+                    "fd_checked_params": len(rows), "fd_seqs": len(sub),
+                    "fd_nonzero": int(sum(abs(fd) > 1e-9 for _, fd in rows)),
+                    "max_abs_err": float(max(err)),
+                    "max_rel_err_nonzero": float(max([e / abs(fd) for e, (_, fd) in zip(err, rows) if abs(fd) > 1e-3]
+                                                     or [0.0]))})
+    elif name == "c5":
+        from contrafold_gpu_amd import serving
+        out.update(serving.benchmark(eng, P, n=args.serve_n, log=log))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2 (default) is the bench line; c1/c3/c4/c5 print their own measurements")
+    ap.add_argument("--check", type=int, default=64, help="sequences the oracle re-folds for c1/c4")
+    ap.add_argument("--serve-n", type=int, default=12000)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -83,6 +180,9 @@ def main():
         print("bench.py measures one GPU: every fold is independent and this build runs on a single MI355X; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.config != "c2":
+        run_config(args.config, args)
+        return
 
     import contrafold_gpu_amd as cf
     from contrafold_gpu_amd import workcount, workloads
@@ -162,13 +262,13 @@ def main():
     }
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, all host cores) ...")
-        cb = cpu_baseline(seqs, masks, P)
+        cb = cpu_baseline(seqs, masks, P, args.samples)
         out["cpu_baseline"] = cb
         out["mea"]["cpu_folds_per_s"] = cb["mea_folds_per_s"]
         out["mea"]["vs_cpu_all_cores"] = round(mea_fps / cb["mea_folds_per_s"], 2)
     if args.serve:
         from contrafold_gpu_amd import serving
-        out["serving"] = serving.benchmark(eng, P, log=log)
+        out["serving"] = serving.benchmark(eng, P, n=args.serve_n, log=log)
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,105 @@
+"""Autotuner for length buckets and workgroup sizes.
+
+A tuning is a list of length edges and one workgroup size per bucket
+(cf_set_tuning).  ``tune`` times each candidate on a sample of the workload
+with the requested stages and installs the fastest; results are cached per
+(device, workload length profile, stages) in a JSON file ($CF_TUNE_CACHE or
+~/.cache/contrafold_gpu_amd/tuning.json).
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+BIG = 1 << 30
+CANDIDATES = [
+    ([64, 128, 256, BIG], [64, 128, 256, 256]),
+    ([64, 160, BIG], [64, 128, 256]),
+    ([96, 192, BIG], [64, 128, 192]),
+    ([128, BIG], [64, 128]),
+    ([128, BIG], [128, 256]),
+    ([BIG], [64]),
+    ([BIG], [128]),
+    ([BIG], [256]),
+    ([80, 200, 320, BIG], [64, 128, 192, 256]),
+]
+DEFAULT = CANDIDATES[0]
+
+
+def cache_path():
+    return os.environ.get("CF_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "contrafold_gpu_amd",
+                                                        "tuning.json"))
+
+
+def profile_key(lengths, flags):
+    L = np.asarray(lengths)
+    hist = np.histogram(L, bins=[0, 64, 128, 192, 256, 384, 512, 768, 1024, 1 << 20])[0]
+    frac = np.round(hist / max(1, L.size), 1)
+    return f"flags={int(flags)};" + ",".join(f"{x:.1f}" for x in frac)
+
+
+def _load_cache():
+    try:
+        with open(cache_path()) as fh:
+            return json.load(fh)
+    except Exception:
+        return {}
+
+
+def _save_cache(data):
+    p = cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        tmp = p + ".tmp"
+        with open(tmp, "w") as fh:
+            json.dump(data, fh, indent=1)
+        os.replace(tmp, p)
+    except OSError:
+        pass
+
+
+def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, sample=1500, repeats=2,
+         candidates=None, use_cache=True, seed=0, log=None):
+    """Pick (edges, threads) for this workload; installs and returns it."""
+    from ._lib import CF_BPP, CF_MEA, CF_SAMPLE
+    flags = CF_BPP if flags is None else flags
+    rng = np.random.default_rng(seed)
+    n = len(seqs)
+    pick = rng.choice(n, size=min(sample, n), replace=False)
+    sub = [seqs[k] for k in pick]
+    subm = [masks[k] for k in pick] if masks is not None else None
+    key = profile_key([len(s) for s in sub], flags)
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        edges, threads = cache[key]["edges"], cache[key]["threads"]
+        engine.set_tuning(edges, threads)
+        return {"edges": edges, "threads": threads, "ms": cache[key].get("ms"), "cached": True}
+    best = None
+    results = []
+    for edges, threads in (candidates or CANDIDATES):
+        engine.set_tuning(edges, threads)
+        with engine.batch(sub, subm) as b:
+            kw = dict(recompute=True)
+            if flags & CF_MEA:
+                kw["gammas"] = list(gammas)
+            if flags & CF_SAMPLE:
+                kw["n_samples"] = n_samples
+            b.run(flags, **kw)
+            t = []
+            for _ in range(repeats):
+                b.run(flags, **kw)
+                t.append(b.last_ms())
+        ms = float(np.median(t))
+        results.append((ms, edges, threads))
+        if log:
+            log(f"[tune] edges={edges} threads={threads}: {ms:.2f} ms")
+        if best is None or ms < best[0]:
+            best = (ms, edges, threads)
+    ms, edges, threads = best
+    engine.set_tuning(edges, threads)
+    if use_cache:
+        cache[key] = {"edges": edges, "threads": threads, "ms": ms, "time": time.time()}
+        _save_cache(cache)
+    return {"edges": edges, "threads": threads, "ms": ms, "cached": False,
+            "tried": [{"ms": m, "edges": e, "threads": t} for m, e, t in results]}

@@ -130,6 +130,7 @@ __constant__ const float cf_exp_coef[6][4] = {
     {0.0573782771f, 0.3580258429f, 0.9121133217f, 0.9793091728f},
     {0.1199175927f, 0.4815668234f, 0.9975991939f, 0.9999505077f},
 };
+__device__ __attribute__((noinline)) float cf_fexp_tail(float x) { return (x > 46.052f) ? 1e20f : expf(x); }
 __device__ __forceinline__ float cf_fexp(float x, const float4* __restrict__ E) {
     int seg = (x >= -5.8622823336f ? 1 : 0) + (x >= -3.8396630909f ? 1 : 0) + (x >= -2.4915033807f ? 1 : 0) +
               (x >= -1.4805375919f ? 1 : 0) + (x >= -0.6725053211f ? 1 : 0);
@@ -141,7 +142,7 @@ __device__ __forceinline__ float cf_fexp(float x, const float4* __restrict__ E) 
     r = r * x;
     r = r + c.w;
     if (x < -9.91152f) r = 0.0f;
-    if (x >= 0.0f) r = (x > 46.052f) ? 1e20f : expf(x);
+    if (__builtin_expect(x >= 0.0f, 0)) r = cf_fexp_tail(x);
     return r;
 }
 

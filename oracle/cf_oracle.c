@@ -758,6 +758,179 @@ float cfo_posterior32(const int32_t *s, int L, const int32_t *fo, const float *P
     return Z;
 }
 
+
+/* ------------------------------------------------------------- sampling
+ * Boltzmann stochastic traceback, restating the reference's sample_kernel
+ * (gpu.py:196-477): explicit stack, per state pass A (max), pass B (sum of
+ * expf(w - max)), pass C (first cumulative >= u * sum), same candidate order.
+ * The random stream is a splitmix64 sequence per (seed, sequence, sample);
+ * the reference's xoroshiro128p stream is parity unpinned (statistics only).
+ * Tables are the FP32 inside tables of cfo_posterior32 (dense (L+2)^2).
+ * out: n_samples rows of L dot-bracket bytes. */
+static inline uint64_t cfo_splitmix(uint64_t *x) {
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static inline float cfo_uniform(uint64_t *x) { return (float)((cfo_splitmix(x) >> 40) + 0.5) * (1.0f / 16777216.0f); }
+
+void cfo_sample32(const int32_t *s, int L, const int32_t *fo, const float *P, const float *FC, const float *FM,
+                  const float *FM1, const float *F5, int n_samples, uint64_t seed, char *out) {
+    const int n = L + 2;
+    const float mb = P[O_MB], mu = P[O_MU], mp = P[O_MP], eu = P[O_EU], ep = P[O_EP];
+    int cap = 4 * L + 16;
+    int *st_t = (int *)malloc(sizeof(int) * cap), *st_i = (int *)malloc(sizeof(int) * cap),
+        *st_j = (int *)malloc(sizeof(int) * cap);
+    for (int smp = 0; smp < n_samples; ++smp) {
+        uint64_t rs = seed * 0x100000001B3ull + (uint64_t)smp * 0x9E3779B97F4A7C15ull;
+        char *o = out + (size_t)smp * L;
+        for (int t = 0; t < L; ++t) o[t] = '.';
+        int sp = 0;
+        st_t[0] = 0; st_i[0] = 0; st_j[0] = L; sp = 1;
+        while (sp > 0) {
+            --sp;
+            int typ = st_t[sp], i = st_i[sp], j = st_j[sp];
+            if (typ == 0) {
+                if (j == 0) continue;
+                float m = NEG32, wun = NEG32;
+                if (F5[j - 1] > HALF32) { wun = F5[j - 1] + eu; if (wun > m) m = wun; }
+#define F5W(k) ({ float jA_ = HC(s[j], s[(k) + 1]); if (j < L) jA_ += DL(s[j], s[(k) + 1], s[j + 1]); \
+                  if ((k) > 0) jA_ += DR(s[j], s[(k) + 1], s[(k)]); \
+                  F5[(k)] + T(FC, (k) + 1, j - 1) + ep + BP(s[(k) + 1], s[j]) + jA_; })
+#define F5OK(k) (CANON[s[(k) + 1]][s[j]] == 1 && fo[(k) + 1] == 0 && fo[j] == 0 && T(FC, (k) + 1, j - 1) > HALF32 && F5[(k)] > HALF32)
+                for (int k = 0; k < j; ++k) if (F5OK(k)) { float w = F5W(k); if (w > m) m = w; }
+                float S = 0.0f;
+                if (wun > HALF32) S += expf(wun - m);
+                for (int k = 0; k < j; ++k) if (F5OK(k)) S += expf(F5W(k) - m);
+                float r = cfo_uniform(&rs) * S, c = 0.0f;
+                int picked = -2;
+                if (wun > HALF32) { c += expf(wun - m); if (r <= c) picked = -1; }
+                if (picked == -2)
+                    for (int k = 0; k < j; ++k) if (F5OK(k)) { c += expf(F5W(k) - m); if (r <= c) { picked = k; break; } }
+                if (picked == -1) { st_t[sp] = 0; st_i[sp] = 0; st_j[sp] = j - 1; ++sp; }
+                else if (picked >= 0) {
+                    int k = picked;
+                    o[k] = '('; o[j - 1] = ')';
+                    st_t[sp] = 1; st_i[sp] = k + 1; st_j[sp] = j - 1; ++sp;
+                    st_t[sp] = 0; st_i[sp] = 0; st_j[sp] = k; ++sp;
+                }
+#undef F5W
+#undef F5OK
+            } else if (typ == 1) {
+                float jB = HC(s[i], s[j + 1]) + TM(s[i], s[j + 1], s[i + 1], s[j]);
+                int d = j - i;
+                float whp = jB + HPC(d <= DHP ? d : DHP);
+                float m = whp;
+                int pmax = i + CMAX; if (pmax > j) pmax = j;
+                float jA = HC(s[i], s[j + 1]);
+                if (i < L) jA += DL(s[i], s[j + 1], s[i + 1]);
+                if (j > 0) jA += DR(s[i], s[j + 1], s[j]);
+                /* candidate weights in order: single-branch (p asc, q desc), then multi k asc */
+                float S = 0.0f, r = 0.0f, c = 0.0f;
+                for (int pass = 0; pass < 3; ++pass) {
+                    if (pass == 1) S = expf(whp - m);
+                    if (pass == 2) { r = cfo_uniform(&rs) * S; c = expf(whp - m); if (r <= c) break; }
+                    int done = 0;
+                    for (int p = i; p <= pmax && !done; ++p) {
+                        int l1 = p - i, qmin = p + 2, alt = p - i + j - CMAX;
+                        if (alt > qmin) qmin = alt;
+                        for (int q = j; q >= qmin; --q) {
+                            int l2 = j - q;
+                            if (!(CANON[s[p + 1]][s[q]] == 1 && fo[p + 1] == 0 && fo[q] == 0 && T(FC, p + 1, q - 1) > HALF32)) continue;
+                            float e;
+                            if (p == i && q == j) e = BP(s[i + 1], s[j]) + STACK(s[i], s[j + 1], s[i + 1], s[j]);
+                            else {
+                                float jB2 = HC(s[q], s[p + 1]) + TM(s[q], s[p + 1], s[q + 1], s[p]);
+                                float sn = 0.0f;
+                                if (l1 == 0 && l2 == 1) sn = B01(s[j]);
+                                else if (l1 == 1 && l2 == 0) sn = B01(s[i + 1]);
+                                else if (l1 == 1 && l2 == 1) sn = I11(s[i + 1], s[j]);
+                                e = CS(l1, l2) + BP(s[p + 1], s[q]) + jB + jB2 + sn;
+                            }
+                            float w = e + T(FC, p + 1, q - 1);
+                            if (pass == 0) { if (w > m) m = w; }
+                            else if (pass == 1) S += expf(w - m);
+                            else {
+                                c += expf(w - m);
+                                if (r <= c) {
+                                    o[p] = '('; o[q - 1] = ')';
+                                    st_t[sp] = 1; st_i[sp] = p + 1; st_j[sp] = q - 1; ++sp;
+                                    done = 1; break;
+                                }
+                            }
+                        }
+                    }
+                    if (done) break;
+                    for (int k = i + 1; k < j; ++k) {
+                        if (!(T(FM1, i, k) > HALF32 && T(FM, k, j) > HALF32)) continue;
+                        float w = T(FM1, i, k) + T(FM, k, j) + jA + mp + mb;
+                        if (pass == 0) { if (w > m) m = w; }
+                        else if (pass == 1) S += expf(w - m);
+                        else {
+                            c += expf(w - m);
+                            if (r <= c) {
+                                st_t[sp] = 3; st_i[sp] = i; st_j[sp] = k; ++sp;
+                                st_t[sp] = 2; st_i[sp] = k; st_j[sp] = j; ++sp;
+                                break;
+                            }
+                        }
+                    }
+                }
+            } else if (typ == 2) {
+                float m = NEG32, wun = NEG32, w1 = NEG32;
+                for (int k = i + 1; k < j; ++k)
+                    if (T(FM1, i, k) > HALF32 && T(FM, k, j) > HALF32) { float w = T(FM1, i, k) + T(FM, k, j); if (w > m) m = w; }
+                if (T(FM, i, j - 1) > HALF32) { wun = T(FM, i, j - 1) + mu; if (wun > m) m = wun; }
+                if (T(FM1, i, j) > HALF32) { w1 = T(FM1, i, j); if (w1 > m) m = w1; }
+                float S = 0.0f;
+                for (int k = i + 1; k < j; ++k)
+                    if (T(FM1, i, k) > HALF32 && T(FM, k, j) > HALF32) S += expf(T(FM1, i, k) + T(FM, k, j) - m);
+                if (wun > HALF32) S += expf(wun - m);
+                if (w1 > HALF32) S += expf(w1 - m);
+                float r = cfo_uniform(&rs) * S, c = 0.0f;
+                int done = 0;
+                for (int k = i + 1; k < j; ++k)
+                    if (T(FM1, i, k) > HALF32 && T(FM, k, j) > HALF32) {
+                        c += expf(T(FM1, i, k) + T(FM, k, j) - m);
+                        if (r <= c) {
+                            st_t[sp] = 3; st_i[sp] = i; st_j[sp] = k; ++sp;
+                            st_t[sp] = 2; st_i[sp] = k; st_j[sp] = j; ++sp;
+                            done = 1; break;
+                        }
+                    }
+                if (!done && wun > HALF32) { c += expf(wun - m); if (r <= c) { st_t[sp] = 2; st_i[sp] = i; st_j[sp] = j - 1; ++sp; done = 1; } }
+                if (!done && w1 > HALF32) { st_t[sp] = 3; st_i[sp] = i; st_j[sp] = j; ++sp; }
+            } else {
+                float wfc = NEG32, wun = NEG32;
+                if (CANON[s[i + 1]][s[j]] == 1 && fo[i + 1] == 0 && fo[j] == 0 && T(FC, i + 1, j - 1) > HALF32) {
+                    float jAji = HC(s[j], s[i + 1]);
+                    if (j < L) jAji += DL(s[j], s[i + 1], s[j + 1]);
+                    if (i > 0) jAji += DR(s[j], s[i + 1], s[i]);
+                    wfc = T(FC, i + 1, j - 1) + jAji + mp + BP(s[i + 1], s[j]);
+                }
+                if (T(FM1, i + 1, j) > HALF32) wun = T(FM1, i + 1, j) + mu;
+                float m = wfc > wun ? wfc : wun, S = 0.0f;
+                if (wfc > HALF32) S += expf(wfc - m);
+                if (wun > HALF32) S += expf(wun - m);
+                float r = cfo_uniform(&rs) * S, c = 0.0f;
+                int done = 0;
+                if (wfc > HALF32) {
+                    c += expf(wfc - m);
+                    if (r <= c) {
+                        o[i] = '('; o[j - 1] = ')';
+                        st_t[sp] = 1; st_i[sp] = i + 1; st_j[sp] = j - 1; ++sp;
+                        done = 1;
+                    }
+                }
+                if (!done && wun > HALF32) { st_t[sp] = 3; st_i[sp] = i + 1; st_j[sp] = j; ++sp; }
+            }
+            (void)n;
+        }
+    }
+    free(st_t); free(st_i); free(st_j);
+}
+
 /* cpu.py:823-895.  POST is (L+2)x(L+2) row-major (as from cfo_posterior32).
  * Fills pair[0..L+1]; optionally returns score[0][L] through *obj. */
 void cfo_mea_decode32(const float *POST, int L, const int32_t *fo, const int32_t *s, float gamma, int32_t *pair,
@@ -832,13 +1005,15 @@ void cfo_mea_decode32(const float *POST, int L, const int32_t *fo, const int32_t
  * per sequence starting at off[b]; the per-sequence outputs are written at
  * b (logZ) or off[b] (pairs, L+2 entries).
  * kind: 0 = FP64 logZ (inside64), 1 = FP32 bpp (posterior32, returns Z),
- *       2 = FP32 MEA (posterior32 + mea_decode32), 3 = FP64 Viterbi. */
+ *       2 = FP32 MEA (posterior32 + mea_decode32), 3 = FP64 Viterbi,
+ *       4 = FP32 bpp + n_samples Boltzmann samples (cfo_sample32). */
 typedef struct {
     int kind, n;
     const int32_t *codes, *masks, *off, *len;
     const double *P64;
     const float *P32;
     float gamma;
+    int n_samples;
     double *out_val;
     int32_t *out_pair;
     int next;
@@ -847,7 +1022,7 @@ typedef struct {
 
 static void *cfo_worker(void *arg) {
     cfo_job *J = (cfo_job *)arg;
-    float *post = NULL;
+    float *post = NULL, *tfc = NULL, *tfm = NULL, *tfm1 = NULL, *tf5 = NULL;
     size_t post_cap = 0;
     for (;;) {
         int b = __atomic_fetch_add(&J->next, 1, __ATOMIC_RELAXED);
@@ -863,20 +1038,45 @@ static void *cfo_worker(void *arg) {
             J->out_val[b] = sc;
         } else {
             size_t need = (size_t)(L + 2) * (L + 2);
-            if (need > post_cap) { free(post); post = (float *)malloc(sizeof(float) * need); post_cap = need; }
-            float z = cfo_posterior32(s, L, fo, J->P32, post, NULL);
-            J->out_val[b] = z;
-            if (J->kind == 2) cfo_mea_decode32(post, L, fo, s, J->gamma, J->out_pair + J->off[b], NULL);
+            if (need > post_cap) {
+                free(post); free(tfc); free(tfm); free(tfm1);
+                post = (float *)malloc(sizeof(float) * need);
+                tfc = (float *)malloc(sizeof(float) * need);
+                tfm = (float *)malloc(sizeof(float) * need);
+                tfm1 = (float *)malloc(sizeof(float) * need);
+                post_cap = need;
+                free(tf5); tf5 = (float *)malloc(sizeof(float) * (L + 2));
+            }
+            if (J->kind == 4) {
+                cfo_tables32 tb; memset(&tb, 0, sizeof(tb));
+                tb.FC = tfc; tb.FM = tfm; tb.FM1 = tfm1;
+                float *f5 = (float *)malloc(sizeof(float) * (L + 2));
+                tb.F5 = f5;
+                float z = cfo_posterior32(s, L, fo, J->P32, post, &tb);
+                J->out_val[b] = z;
+                if (J->n_samples > 0 && L > 0) {
+                    char *buf = (char *)malloc((size_t)J->n_samples * L);
+                    cfo_sample32(s, L, fo, J->P32, tfc, tfm, tfm1, f5, J->n_samples, (uint64_t)b + 1, buf);
+                    J->out_pair[J->off[b]] = buf[0];
+                    free(buf);
+                }
+                free(f5);
+            } else {
+                float z = cfo_posterior32(s, L, fo, J->P32, post, NULL);
+                J->out_val[b] = z;
+                if (J->kind == 2) cfo_mea_decode32(post, L, fo, s, J->gamma, J->out_pair + J->off[b], NULL);
+            }
         }
     }
-    free(post);
+    free(post); free(tfc); free(tfm); free(tfm1); free(tf5);
     return NULL;
 }
 
 int cfo_batch(int kind, int n, const int32_t *codes, const int32_t *masks, const int32_t *off, const int32_t *len,
-              const double *P64, const float *P32, float gamma, double *out_val, int32_t *out_pair, int threads) {
+              const double *P64, const float *P32, float gamma, double *out_val, int32_t *out_pair, int threads,
+              int n_samples) {
     cfo_job J;
-    J.kind = kind; J.n = n; J.codes = codes; J.masks = masks; J.off = off; J.len = len;
+    J.kind = kind; J.n = n; J.codes = codes; J.masks = masks; J.off = off; J.len = len; J.n_samples = n_samples;
     J.P64 = P64; J.P32 = P32; J.gamma = gamma; J.out_val = out_val; J.out_pair = out_pair; J.next = 0;
     if (threads < 1) threads = 1;
     pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * threads);

@@ -88,7 +88,10 @@ def lib():
         L.cfo_fexp32.argtypes = [ctypes.c_float]
         L.cfo_batch.restype = ctypes.c_int
         L.cfo_batch.argtypes = [ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, i32p, f64p, f32p,
-                                ctypes.c_float, f64p, i32p, ctypes.c_int]
+                                ctypes.c_float, f64p, i32p, ctypes.c_int, ctypes.c_int]
+        L.cfo_sample32.restype = None
+        L.cfo_sample32.argtypes = [i32p, ctypes.c_int, i32p, f32p, f32p, f32p, f32p, f32p, ctypes.c_int,
+                                   ctypes.c_uint64, ctypes.c_char_p]
         L.cfo_table_size.restype = ctypes.c_int
         assert L.cfo_table_size() == N_TABLE
         _lib = L
@@ -315,6 +318,20 @@ def mfe(seq, P, forced=None):
     return dot_bracket(pair, len(seq))
 
 
+def sample32(seq, P, n_samples, seed=0, forced=None):
+    """CPU Boltzmann samples (restated 3-pass traceback of gpu.py:196-477)."""
+    s, L = encode(seq)
+    fo = encode_mask(L, forced)
+    _z, _POST, T = posterior32(seq, P, forced, tables=True)
+    buf = ctypes.create_string_buffer(max(1, n_samples * L))
+    v = flat32(P)
+    lib().cfo_sample32(_p(s, ctypes.c_int32), L, _p(fo, ctypes.c_int32), _p(v, ctypes.c_float),
+                       _p(T["FC"], ctypes.c_float), _p(T["FM"], ctypes.c_float), _p(T["FM1"], ctypes.c_float),
+                       _p(T["F5"], ctypes.c_float), int(n_samples), int(seed), buf)
+    raw = buf.raw[:n_samples * L].decode("ascii")
+    return [raw[k * L:(k + 1) * L] for k in range(n_samples)]
+
+
 def lse32(a, b):
     return np.float32(lib().cfo_lse32(ctypes.c_float(np.float32(a)), ctypes.c_float(np.float32(b))))
 
@@ -324,7 +341,7 @@ def fexp32(x):
 
 
 # ----------------------------------------------------------------- batch runner
-KIND = {"logz64": 0, "bpp32": 1, "mea32": 2, "viterbi64": 3}
+KIND = {"logz64": 0, "bpp32": 1, "mea32": 2, "viterbi64": 3, "bpp_sample32": 4}
 
 
 def pack(seqs, masks=None):
@@ -344,7 +361,7 @@ def pack(seqs, masks=None):
     return codes, fo, off, lens
 
 
-def run_batch(kind, seqs, P, masks=None, gamma=6.0, threads=None):
+def run_batch(kind, seqs, P, masks=None, gamma=6.0, threads=None, n_samples=0):
     """Fold ``seqs`` on ``threads`` host threads.  Returns (values float64[n], pairs list or None)."""
     codes, fo, off, lens = pack(seqs, masks)
     n = len(seqs)
@@ -356,7 +373,7 @@ def run_batch(kind, seqs, P, masks=None, gamma=6.0, threads=None):
     lib().cfo_batch(KIND[kind], n, _p(codes, ctypes.c_int32), _p(fo, ctypes.c_int32), _p(off, ctypes.c_int32),
                     _p(lens, ctypes.c_int32), _p(v64, ctypes.c_double), _p(v32, ctypes.c_float),
                     ctypes.c_float(np.float32(gamma)), _p(out_val, ctypes.c_double), _p(out_pair, ctypes.c_int32),
-                    int(threads))
+                    int(threads), int(n_samples))
     pairs = None
     if kind in ("mea32", "viterbi64"):
         pairs = [out_pair[off[b]:off[b] + lens[b] + 2].copy() for b in range(n)]

@@ -242,3 +242,59 @@ def test_server_admission_budget(engine, P):
     srv.close()
     assert done[big] is None
     assert all(done[i] is not None for i in ids)
+
+
+def test_autotune_installs_fastest(engine, P, tmp_path, monkeypatch):
+    from contrafold_gpu_amd import autotune, CF_BPP
+    monkeypatch.setenv("CF_TUNE_CACHE", str(tmp_path / "tune.json"))
+    rng = np.random.default_rng(17)
+    seqs = [rand_seq(rng, int(rng.integers(30, 300))) for _ in range(60)]
+    cands = autotune.CANDIDATES[:3]
+    res = autotune.tune(engine, seqs, flags=CF_BPP, sample=60, repeats=1, candidates=cands)
+    assert not res["cached"] and len(res["tried"]) == 3
+    assert res["ms"] == min(t["ms"] for t in res["tried"])
+    again = autotune.tune(engine, seqs, flags=CF_BPP, sample=60, repeats=1, candidates=cands)
+    assert again["cached"] and again["edges"] == res["edges"]
+    engine.set_tuning(*autotune.DEFAULT)
+
+
+def test_cli_modes(P, tmp_path, capsys):
+    import json
+    from contrafold_gpu_amd import cli
+    import contrafold_gpu_amd as cf
+    cli.main(["GGGGAAAACCCC"])
+    assert capsys.readouterr().out.strip() == "((((....))))"
+    cli.main(["GGGGAAAACCCC", "--viterbi"])
+    assert capsys.readouterr().out.strip() == "((((....))))"
+    p = tmp_path / "in.jsonl"
+    p.write_text(json.dumps({"id": "a", "seq": "GCGCGCAAAAGCGCGC"}) + "\n" +
+                 json.dumps({"id": "b", "seq": "GGGAAACCC", "constrain": "000010000"}) + "\n")
+    o = tmp_path / "out.jsonl"
+    cli.main([str(p), "-o", str(o)])
+    recs = [json.loads(x) for x in o.read_text().splitlines()]
+    assert recs[0]["structure"] == cf.mea("GCGCGCAAAAGCGCGC", P)
+    assert recs[1]["structure"][4] == "."
+    cli.main([str(p), "--logz", "-o", str(o)])
+    recs = [json.loads(x) for x in o.read_text().splitlines()]
+    assert abs(recs[0]["logZ"] - cf.logZ_batch(["GCGCGCAAAAGCGCGC"], P)[0]) < 1e-6
+    cli.main([str(p), "--sample", "3", "-o", str(o)])
+    recs = [json.loads(x) for x in o.read_text().splitlines()]
+    assert len(recs[1]["samples"]) == 3
+
+
+def test_uneven_lengths_buckets_and_n_bases(engine, P, oracle):
+    """Uneven-length batch with 2% N (workload C3 shape, small): every bucket's
+    MEA at three gammas and Viterbi equal the oracle."""
+    from contrafold_gpu_amd import workloads, CF_MEA, CF_MFE
+    seqs = workloads.uneven(24, lo=30, hi=300, seed=3)
+    gammas = [0.5, 6.0, 16.0]
+    with engine.batch(seqs) as b:
+        b.run(CF_MEA | CF_MFE, gammas=gammas)
+        got = [b.mea(g) for g in range(3)]
+        vit, _ = b.mfe()
+    for k, s in enumerate(seqs):
+        _z, POST = oracle.posterior32(s, P)
+        for g_i, g in enumerate(gammas):
+            pair, _ = oracle.mea_decode32(POST, s, g)
+            assert got[g_i][k] == oracle.dot_bracket(pair, len(s))
+        assert vit[k] == oracle.mfe(s, P)

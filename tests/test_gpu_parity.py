@@ -195,3 +195,19 @@ def test_long_sequence_parity(engine, P, oracle):
         got = b.bpp(0)
         assert (got.view(np.uint32) == POST[1:421, 1:421].view(np.uint32)).all()
         assert b.mea(0)[0] == oracle.mea(s, P, 6.0)
+
+
+def test_beyond_lds_ring_parity(engine, P, oracle):
+    """L = 1100: the FC/FCo ring no longer fits in LDS, so the kernels read the
+    window from the global tables (the long-sequence path)."""
+    from contrafold_gpu_amd import CF_MEA
+    rng = np.random.default_rng(1100)
+    s = rand_seq(rng, 1100)
+    with engine.batch([s]) as b:
+        b.run(CF_MEA, gammas=[6.0])
+        z32, POST = oracle.posterior32(s, P)
+        assert b.logz()[0].view(np.uint32) == np.float32(z32).view(np.uint32)
+        got = b.bpp(0)
+        assert (got.view(np.uint32) == POST[1:1101, 1:1101].view(np.uint32)).all()
+        pair, _ = oracle.mea_decode32(POST, s, 6.0)
+        assert b.mea(0)[0] == oracle.dot_bracket(pair, 1100)
