@@ -14,15 +14,16 @@ import numpy as np
 
 BIG = 1 << 30
 CANDIDATES = [
-    ([64, 128, 256, BIG], [64, 128, 256, 256]),
-    ([64, 160, BIG], [64, 128, 256]),
-    ([96, 192, BIG], [64, 128, 192]),
-    ([128, BIG], [64, 128]),
-    ([128, BIG], [128, 256]),
+    # width 64 = packed (one wave per sequence, 4 sequences per workgroup);
+    # wider = the whole workgroup on one sequence
+    ([128, 256, BIG], [64, 64, 64]),
     ([BIG], [64]),
+    ([384, BIG], [64, 256]),
+    ([512, BIG], [64, 256]),
+    ([64, 128, 256, BIG], [64, 128, 256, 256]),
+    ([128, BIG], [128, 256]),
     ([BIG], [128]),
     ([BIG], [256]),
-    ([80, 200, 320, BIG], [64, 128, 192, 256]),
 ]
 DEFAULT = CANDIDATES[0]
 

@@ -131,6 +131,40 @@ def fold(records, P, mode="mea", gamma=6.0, sample_n=0, with_logz=False, chunk=4
     return out
 
 
+def write_posteriors(records, P, cutoff, out):
+    """CONTRAfold --posteriors text: one line per position i (1-based),
+    'i base j:p ...' listing every partner j (either side) with p >= cutoff;
+    records are separated by a '>id' line.  The reference parses exactly this
+    (gpu_contrafold/reference.py:103-132)."""
+    import contrafold_gpu_amd as cf
+    masks = [build_mask(r[1], r[2], where=f"id={r[0]!r}: ") for r in records]
+    mats = cf.bpp_batch([r[1] for r in records], P, masks)
+    for (rid, seq, _c), M in zip(records, mats):
+        full = M + M.T
+        if len(records) > 1:
+            out.write(f">{rid}\n")
+        for i in range(len(seq)):
+            js = np.nonzero(full[i] >= cutoff)[0]
+            toks = " ".join(f"{j + 1}:{full[i, j]:.6g}" for j in js if j != i)
+            out.write(f"{i + 1} {seq[i]}" + (" " + toks if toks else "") + "\n")
+
+
+def parse_posteriors(text, L):
+    """Dense L x L upper-triangular matrix from CONTRAfold --posteriors text."""
+    mat = np.zeros((L, L))
+    for line in text.splitlines():
+        parts = line.split()
+        if len(parts) < 2 or line.startswith(">"):
+            continue
+        i = int(parts[0]) - 1
+        for tok in parts[2:]:
+            j_s, p_s = tok.split(":")
+            j = int(j_s) - 1
+            a, b = (i, j) if j > i else (j, i)
+            mat[a, b] = float(p_s)
+    return mat
+
+
 def main(argv=None):
     argv = list(sys.argv[1:] if argv is None else argv)
     if argv and argv[0] == "fold":
@@ -146,13 +180,33 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=4096, help="sequences per GPU batch")
     ap.add_argument("--threads", type=int, default=128, help="accepted for compatibility; ignored")
     ap.add_argument("--seed", type=int, default=0, help="RNG seed for sampling")
+    ap.add_argument("--posteriors", type=float, default=None, metavar="CUTOFF",
+                    help="write base-pair posteriors >= CUTOFF in CONTRAfold's sparse format "
+                         "('i base j:p ...'), as `contrafold predict --posteriors`")
+    ap.add_argument("--parens", action="store_true",
+                    help="write structures as CONTRAfold --parens records (>id / sequence / structure)")
+    ap.add_argument("--partition", action="store_true",
+                    help="print CONTRAfold --partition lines (Log partition coefficient for \"id\": X)")
     args = ap.parse_args(argv)
     records, from_file = load_records(args.input)
     if not records:
         raise SystemExit("no sequences in input")
-    mode = "sample" if args.sample else ("logz" if args.logz else ("mfe" if args.viterbi else "mea"))
     import contrafold_gpu_amd as cf
     P = cf.load()
+    if args.posteriors is not None or args.partition:
+        out = open(args.output, "w") if args.output else sys.stdout
+        try:
+            if args.posteriors is not None:
+                write_posteriors(records, P, args.posteriors, out)
+            else:
+                res = fold(records, P, mode="logz", chunk=args.chunk)
+                for k, (rid, _s, _c) in enumerate(records):
+                    out.write(f'Log partition coefficient for "{rid}": {res["logZ"][k]:.6g}\n')
+        finally:
+            if args.output:
+                out.close()
+        return
+    mode = "sample" if args.sample else ("logz" if args.logz else ("mfe" if args.viterbi else "mea"))
     res = fold(records, P, mode=mode, gamma=args.gamma, sample_n=args.sample,
                with_logz=args.logz and mode == "sample", chunk=args.chunk, seed=args.seed)
     if not from_file and not args.output:
@@ -167,6 +221,14 @@ def main(argv=None):
             print(f"{res['logZ'][0]:.6f}")
         return
     out = open(args.output, "w") if args.output else sys.stdout
+    if args.parens and mode in ("mfe", "mea"):
+        try:
+            for k, (rid, seq, _c) in enumerate(records):
+                out.write(f">{rid}\n{seq}\n{res['structure'][k]}\n")
+        finally:
+            if args.output:
+                out.close()
+        return
     try:
         for k, (rid, _s, _c) in enumerate(records):
             rec = {"id": rid}

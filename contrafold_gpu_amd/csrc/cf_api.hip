@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/contrafold_amd.h"
+#include "contrafold_amd.h"
 
 #define CF_VERSION 1
 
@@ -91,23 +91,32 @@ int vit_lds(int Lmax) {
 }
 
 // ---------------------------------------------------------------- stages
+// A bucket whose tuned width is one wave (64 threads) runs packed: kPack
+// sequences per workgroup, one wave each, sharing the staged parameter
+// tables.  Wider buckets give the whole workgroup to one sequence.
+constexpr int kPack = 4;
+
 int launch_inside(cf_batch* b) {
     cf_ctx* c = b->ctx;
     for (const auto& bk : b->buckets) {
         const int nl = bk.end - bk.begin;
         if (nl <= 0) continue;
-        CfLds plan = cf_lds_plan(bk.Lmax, true, 4);
-        const bool ring = plan.total <= c->lds_max;
-        if (!ring) plan = cf_lds_plan(bk.Lmax, false, 4);
+        // The inside sweep reads its single-branch window from the global FC
+        // table (L2-resident): without the LDS ring a workgroup needs ~14 KB of
+        // LDS at 200 nt instead of ~40 KB, which doubles the resident waves
+        // per CU and beats the ring on throughput (54 vs 87 ms for 3000 x 200 nt).
+        const bool pack = bk.threads <= 64;
+        const int G = pack ? kPack : 1;
+        const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, G);
         if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the inside kernel's LDS plan");
-        if (ring)
-            hipLaunchKernelGGL(cf_inside_kernel<true>, dim3(nl), dim3(bk.threads), plan.total, b->stream, b->d_seqs,
-                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
-                               b->FCX, bk.Lmax);
+        if (pack)
+            hipLaunchKernelGGL((cf_inside_kernel<false, true>), dim3((nl + G - 1) / G), dim3(64 * G), plan.total,
+                               b->stream, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC,
+                               b->FM, b->FM1, b->FM2, b->FCX, bk.Lmax);
         else
-            hipLaunchKernelGGL(cf_inside_kernel<false>, dim3(nl), dim3(bk.threads), plan.total, b->stream, b->d_seqs,
-                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
-                               b->FCX, bk.Lmax);
+            hipLaunchKernelGGL((cf_inside_kernel<false, false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
+                               b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
+                               b->FM1, b->FM2, b->FCX, bk.Lmax);
         CF_HIP(c, hipGetLastError(), "inside launch");
     }
     return CF_OK;
@@ -127,18 +136,29 @@ int launch_outside(cf_batch* b) {
     for (const auto& bk : b->buckets) {
         const int nl = bk.end - bk.begin;
         if (nl <= 0) continue;
+        if (bk.threads <= 64) {
+            // packed: no room for a per-sequence FCo ring, the window is read from L2
+            const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, kPack);
+            if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
+            hipLaunchKernelGGL((cf_outside_kernel<false, true>), dim3((nl + kPack - 1) / kPack), dim3(64 * kPack),
+                               plan.total, b->stream, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks,
+                               c->dP32, b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST,
+                               bk.Lmax);
+            CF_HIP(c, hipGetLastError(), "outside launch");
+            continue;
+        }
         CfLds plan = cf_lds_plan(bk.Lmax, true, 4);
         const bool ring = plan.total <= c->lds_max;
         if (!ring) plan = cf_lds_plan(bk.Lmax, false, 4);
         if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
         if (ring)
-            hipLaunchKernelGGL(cf_outside_kernel<true>, dim3(nl), dim3(bk.threads), plan.total, b->stream, b->d_seqs,
-                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
-                               b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST, bk.Lmax);
+            hipLaunchKernelGGL((cf_outside_kernel<true, false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
+                               b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
+                               b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST, bk.Lmax);
         else
-            hipLaunchKernelGGL(cf_outside_kernel<false>, dim3(nl), dim3(bk.threads), plan.total, b->stream, b->d_seqs,
-                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
-                               b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST, bk.Lmax);
+            hipLaunchKernelGGL((cf_outside_kernel<false, false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
+                               b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
+                               b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST, bk.Lmax);
         CF_HIP(c, hipGetLastError(), "outside launch");
     }
     return CF_OK;
@@ -593,8 +613,9 @@ int cf_create(int device, cf_ctx** out) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
         c->lds_max = v;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0) c->n_cu = v;
-    const void* dyn[] = {(const void*)cf_inside_kernel<true>, (const void*)cf_inside_kernel<false>,
-                         (const void*)cf_outside_kernel<true>, (const void*)cf_outside_kernel<false>,
+    const void* dyn[] = {(const void*)cf_inside_kernel<false, false>, (const void*)cf_inside_kernel<false, true>,
+                         (const void*)cf_outside_kernel<true, false>, (const void*)cf_outside_kernel<false, false>,
+                         (const void*)cf_outside_kernel<false, true>,
                          (const void*)cf_mea_kernel, (const void*)cf_viterbi_kernel<false>,
                          (const void*)cf_viterbi_kernel<true>};
     for (const void* f : dyn) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);

@@ -8,7 +8,7 @@
 #include <vector>
 
 #include "cf_device.h"
-#include "../../include/contrafold_amd.h"
+#include "contrafold_amd.h"
 
 namespace cfi {
 
@@ -55,8 +55,9 @@ struct cf_ctx {
     CfParams<float>* dP32 = nullptr;
     CfParams<double>* dP64 = nullptr;
     uint64_t params_version = 0;
-    std::vector<int> edges{64, 128, 256, 1 << 30};
-    std::vector<int> threads{64, 128, 256, 256};
+    // bucket edges (max length) and team width per bucket; width 64 = packed
+    std::vector<int> edges{128, 256, 1 << 30};
+    std::vector<int> threads{64, 64, 64};
     int lds_max = 65536;
     int n_cu = 256;
     cfi::DevPool pool;

@@ -24,12 +24,21 @@
 #define WAVE 64
 
 // ------------------------------------------------------------------ LDS plan
+// A workgroup runs G "teams", each folding one sequence.  G = 1 is the wide
+// layout (the whole workgroup on one sequence, __syncthreads between
+// diagonals); G = blockDim/64 is the packed layout (one wave per sequence,
+// the waves of a workgroup share the staged parameter tables and never wait
+// for each other after staging).  LDS = parameter tables once + G team areas.
 struct CfLds {
     int W;  // 32-bit words per partner-mask row
-    int o_lse, o_exp, o_cs, o_jb, o_bp, o_qinfo, o_s8, o_fo8, o_rs8, o_pmask, o_dbuf, o_alist, o_cnt, o_ring, total;
+    // shared area (offsets from the start of LDS)
+    int o_lse, o_exp, o_cs, o_jb, o_bp, o_bj, shared;
+    // team area (offsets from the team's base)
+    int o_qinfo, o_s8, o_fo8, o_rs8, o_pmask, o_dbuf, o_alist, o_cnt, o_ring, team;
+    int G, total;
 };
 __host__ __device__ inline int cf_align(int x, int a) { return (x + a - 1) / a * a; }
-__host__ __device__ inline CfLds cf_lds_plan(int Lmax, bool ring, int dbuf_elem) {
+__host__ __device__ inline CfLds cf_lds_plan(int Lmax, bool ring, int dbuf_elem, int G = 1) {
     CfLds p;
     p.W = ((Lmax + 2 + 31) >> 5) + 1;
     int o = 0;
@@ -39,6 +48,9 @@ __host__ __device__ inline CfLds cf_lds_plan(int Lmax, bool ring, int dbuf_elem)
     p.o_jb = o; o += 625 * 4;
     p.o_bp = o; o += 25 * 4;
     o = cf_align(o, 16);
+    p.o_bj = o; o += 625 * 8;
+    p.shared = cf_align(o, 16);
+    o = 0;
     p.o_qinfo = o; o += (Lmax + 3) * 4;
     p.o_s8 = o; o += Lmax + 3;
     p.o_fo8 = o; o += Lmax + 3;
@@ -53,7 +65,9 @@ __host__ __device__ inline CfLds cf_lds_plan(int Lmax, bool ring, int dbuf_elem)
     p.o_cnt = o; o += 16;
     p.o_ring = o;
     if (ring) o += CF_RING * (Lmax + 1) * dbuf_elem;
-    p.total = cf_align(o, 16);
+    p.team = cf_align(o, 16);
+    p.G = G;
+    p.total = p.shared + G * p.team;
     return p;
 }
 
@@ -63,6 +77,7 @@ struct CfShared {
     float* cs;
     float* JB;
     float* bp;
+    float2* BJ;  // (JB[q][p+1][q+1][p], bp[p+1][q]) for one 8-byte read per candidate
     int* qinfo;
     int8_t* s8;
     int8_t* fo8;
@@ -75,42 +90,71 @@ struct CfShared {
     void* ring;
 };
 
-__device__ inline CfShared cf_carve(unsigned char* smem, const CfLds& p) {
+__device__ inline CfShared cf_carve(unsigned char* smem, const CfLds& p, int team = 0) {
     CfShared S;
     S.lse = (float4*)(smem + p.o_lse);
     S.ex = (float4*)(smem + p.o_exp);
     S.cs = (float*)(smem + p.o_cs);
     S.JB = (float*)(smem + p.o_jb);
     S.bp = (float*)(smem + p.o_bp);
-    S.qinfo = (int*)(smem + p.o_qinfo);
-    S.s8 = (int8_t*)(smem + p.o_s8);
-    S.fo8 = (int8_t*)(smem + p.o_fo8);
-    S.rs8 = (int8_t*)(smem + p.o_rs8);
-    S.pmask = (uint32_t*)(smem + p.o_pmask);
+    S.BJ = (float2*)(smem + p.o_bj);
+    unsigned char* t = smem + p.shared + team * p.team;
+    S.qinfo = (int*)(t + p.o_qinfo);
+    S.s8 = (int8_t*)(t + p.o_s8);
+    S.fo8 = (int8_t*)(t + p.o_fo8);
+    S.rs8 = (int8_t*)(t + p.o_rs8);
+    S.pmask = (uint32_t*)(t + p.o_pmask);
     S.W = p.W;
-    S.dbuf = (void*)(smem + p.o_dbuf);
-    S.alist = (uint16_t*)(smem + p.o_alist);
-    S.cnt = (int*)(smem + p.o_cnt);
-    S.ring = (void*)(smem + p.o_ring);
+    S.dbuf = (void*)(t + p.o_dbuf);
+    S.alist = (uint16_t*)(t + p.o_alist);
+    S.cnt = (int*)(t + p.o_cnt);
+    S.ring = (void*)(t + p.o_ring);
     return S;
 }
 
-// Stage the per-sequence inputs and the hot parameter tables in LDS.
-__device__ void cf_stage(const CfShared& S, const CfParams<float>* __restrict__ P, const uint8_t* __restrict__ codes,
-                         const uint8_t* __restrict__ masks, const CfSeq& sd) {
+// Synchronise the threads of one team.  A packed team is a single wave: its
+// lanes run in lockstep, so ordering its own LDS and global accesses needs
+// only the fences (waits for outstanding LDS operations and keeps the
+// compiler from moving memory accesses across); a wide team is the
+// workgroup.
+template <bool PACK>
+__device__ __forceinline__ void cf_team_sync() {
+    if (PACK) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        __syncthreads();
+    }
+}
+
+// Stage the hot parameter tables in LDS (every thread of the workgroup takes
+// part; the caller synchronises the workgroup afterwards).
+__device__ void cf_stage_params(const CfShared& S, const CfParams<float>* __restrict__ P) {
     const int tid = threadIdx.x, NT = blockDim.x;
-    const int L = sd.L;
     for (int t = tid; t < 8; t += NT) S.lse[t] = make_float4(cf_lse_coef[t][0], cf_lse_coef[t][1], cf_lse_coef[t][2], cf_lse_coef[t][3]);
     for (int t = tid; t < 6; t += NT) S.ex[t] = make_float4(cf_exp_coef[t][0], cf_exp_coef[t][1], cf_exp_coef[t][2], cf_exp_coef[t][3]);
     for (int t = tid; t < 961; t += NT) S.cs[t] = P->t[CF_O_CS + t];
     for (int t = tid; t < 625; t += NT) S.JB[t] = P->JB[t];
     for (int t = tid; t < 25; t += NT) S.bp[t] = P->t[CF_O_BP + t];
+    for (int t = tid; t < 625; t += NT) {
+        // t = ((sq*5 + sp1)*5 + sq1)*5 + sp
+        const int sq = t / 125, sp1 = (t / 25) % 5;
+        S.BJ[t] = make_float2(P->JB[t], P->t[CF_O_BP + cf_i2(sp1, sq)]);
+    }
+}
+
+// Stage one sequence (codes, forced mask, partner masks) in the team's area.
+template <bool PACK>
+__device__ void cf_stage_seq(const CfShared& S, const uint8_t* __restrict__ codes, const uint8_t* __restrict__ masks,
+                             const CfSeq& sd, int tid, int NT) {
+    const int L = sd.L;
     for (int q = tid; q < L + 2; q += NT) {
         S.s8[q] = (int8_t)codes[sd.coff + q];
         S.fo8[q] = (masks != nullptr && masks[sd.coff + q] != 0) ? 1 : 0;
     }
     if (tid == 0) { S.s8[L + 2] = 4; S.fo8[L + 2] = 0; S.cnt[0] = 0; S.cnt[1] = 0; }
-    __syncthreads();
+    cf_team_sync<PACK>();
     for (int q = tid; q < L + 2; q += NT) {
         const int c = S.s8[q];
         S.rs8[q] = (S.fo8[q] || c == 4) ? 4 : c;
@@ -128,7 +172,31 @@ __device__ void cf_stage(const CfShared& S, const CfParams<float>* __restrict__ 
         }
         for (int x = 0; x < 5; ++x) S.pmask[x * S.W + w] = m[x];
     }
-    __syncthreads();
+    cf_team_sync<PACK>();
+}
+
+// Team geometry of a launch: which sequence this thread's team folds, and the
+// thread's rank and count within the team.  The wave index is made
+// wave-uniform explicitly so that everything derived from the sequence
+// descriptor (table bases, lengths, loop bounds) lives in scalar registers.
+struct CfTeam {
+    int team, tid, NT, li;
+};
+template <bool PACK>
+__device__ __forceinline__ CfTeam cf_team(int G) {
+    CfTeam t;
+    if (PACK) {
+        t.team = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+        t.tid = threadIdx.x & (WAVE - 1);
+        t.NT = WAVE;
+        t.li = blockIdx.x * G + t.team;
+    } else {
+        t.team = 0;
+        t.tid = threadIdx.x;
+        t.NT = blockDim.x;
+        t.li = blockIdx.x;
+    }
+    return t;
 }
 
 __device__ __forceinline__ bool cf_allow(const CfShared& S, int a, int b) {
@@ -139,19 +207,25 @@ __device__ __forceinline__ bool cf_allow(const CfShared& S, int a, int b) {
 // FP32 inside sweep (cpu.py:595-665; gpu.py:79-168).  Writes FC, FM1, FM,
 // FM2 (the multiloop split, kept for sampling and gradients) and the exterior
 // copy FCX of FC.  F5 is produced by cf_exterior_kernel.
-template <bool RING>
+template <bool RING, bool PACK>
 __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
-                                                        const uint8_t* __restrict__ codes, const uint8_t* __restrict__ masks,
+                                                        int nseq, const uint8_t* __restrict__ codes,
+                                                        const uint8_t* __restrict__ masks,
                                                         const CfParams<float>* __restrict__ P, float* __restrict__ FC,
                                                         float* __restrict__ FM, float* __restrict__ FM1,
                                                         float* __restrict__ FM2, float* __restrict__ FCX, int Lmax) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const CfLds plan = cf_lds_plan(Lmax, RING, 4);
-    const CfShared S = cf_carve(smem, plan);
-    const CfSeq sd = seqs[order[blockIdx.x]];
+    const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
+    const CfTeam tm = cf_team<PACK>(G);
+    const CfLds plan = cf_lds_plan(Lmax, RING, 4, G);
+    const CfShared S = cf_carve(smem, plan, tm.team);
+    cf_stage_params(S, P);
+    __syncthreads();
+    if (tm.li >= nseq) return;
+    const CfSeq sd = seqs[order[tm.li]];
     const int L = sd.L;
-    const int tid = threadIdx.x, NT = blockDim.x;
-    cf_stage(S, P, codes, masks, sd);
+    const int tid = tm.tid, NT = tm.NT;
+    cf_stage_seq<PACK>(S, codes, masks, sd, tid, NT);
 
     const float mb = P->t[CF_O_MB], mu = P->t[CF_O_MU], mp = P->t[CF_O_MP];
     const float* __restrict__ T = P->t;
@@ -176,14 +250,30 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
             const bool alw = inner && cf_allow(S, i, j + 1);
             float fm2 = CF_NEG;
             if (inner && s >= 4) {
-                // FM2[i][j] = LSE_{k=i+2..j-2} FM1[i][k] + FM[k][j]  (k = i + t)
+                // FM2[i][j] = LSE_{k=i+2..j-2} FM1[i][k] + FM[k][j]  (k = i + t).
+                // Loads of 8 consecutive terms are issued before the 8 folds,
+                // so memory latency overlaps; the fold order is unchanged.
                 int64_t ia = cf_dbase(L, 2) + i;
                 int64_t ib = cf_dbase(L, s - 2) + i + 2;
-                for (int t = 2; t <= s - 2; ++t) {
+                int t = 2;
+                for (; t + 7 <= s - 2; t += 8) {
+                    float x[8], y[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        x[u] = fm1_t[ia];
+                        y[u] = fm_t[ib];
+                        ia += L + 1 - (t + u);
+                        ib -= L + 1 - (s - (t + u) - 1);
+                        ib += 1;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        fm2 = cf_lse(fm2, (x[u] > CF_HALF && y[u] > CF_HALF) ? x[u] + y[u] : CF_NEG, C);
+                }
+                for (; t <= s - 2; ++t) {
                     const float x = fm1_t[ia];
                     const float y = fm_t[ib];
-                    const float v = (x > CF_HALF && y > CF_HALF) ? x + y : CF_NEG;
-                    fm2 = cf_lse(fm2, v, C);
+                    fm2 = cf_lse(fm2, (x > CF_HALF && y > CF_HALF) ? x + y : CF_NEG, C);
                     ia += L + 1 - t;
                     ib -= L + 1 - (s - t - 1);
                     ib += 1;
@@ -224,7 +314,7 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
                 if (inner) fcx_t[cf_xbase(j + 1) + i - 1] = CF_NEG;
             }
         }
-        __syncthreads();
+        cf_team_sync<PACK>();
         // ---- phase 2: FC for the pairable cells (hairpin, single-branch window, multiloop)
         const int nal = S.cnt[s & 1];
         if (tid == 0) S.cnt[(s + 1) & 1] = 0;
@@ -237,38 +327,52 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
             const float e_stack = S.bp[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
             const float sn01 = T[CF_O_B01 + sj], sn10 = T[CF_O_B01 + si1], sn11 = T[CF_O_I11 + cf_i2(si1, sj)];
             const int pend = min(i + CF_CMAX, j - 2);
-            int p = i, qlo = 0, sp1 = 0, rowJ = 0;
+            int p = i, qlo = 0, rowJ = 0;
             uint32_t bits = 0u;
             if (p <= pend) {
                 qlo = max(p + 2, j - CF_CMAX);
-                sp1 = S.s8[p + 1];
-                rowJ = sp1 * 25 + S.s8[p];
+                rowJ = S.s8[p + 1] * 25 + S.s8[p];
                 bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
             }
-            while (true) {
-                while (bits == 0u && p < pend) {
-                    ++p;
-                    qlo = max(p + 2, j - CF_CMAX + (p - i));
-                    sp1 = S.s8[p + 1];
-                    rowJ = sp1 * 25 + S.s8[p];
-                    bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
+            // Candidates (p ascending, q descending) are gathered 8 at a time
+            // -- their LDS reads do not depend on acc -- then folded in order;
+            // exhausted slots hold NEG, which leaves acc unchanged.
+            bool more = (p <= pend);
+            while (more) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    while (bits == 0u && p < pend) {
+                        ++p;
+                        qlo = max(p + 2, j - CF_CMAX + (p - i));
+                        rowJ = S.s8[p + 1] * 25 + S.s8[p];
+                        bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
+                    }
+                    float val = CF_NEG;
+                    if (bits != 0u) {
+                        const int k = 31 - __clz(bits);
+                        bits &= ~(1u << k);
+                        const int q = qlo + k;
+                        const int l1 = p - i, l2 = j - q;
+                        const int dd = s - l1 - l2 - 2;
+                        const float fc = RING ? ring[(dd % CF_RING) * RS + p + 1] : fc_t[cf_dbase(L, dd) + p + 1];
+                        const int qi = S.qinfo[q];
+                        const float2 bj = S.BJ[(qi & 0xffff) + rowJ];
+                        float e = S.cs[l1 * 31 + l2] + bj.y;
+                        e = e + jB;
+                        e = e + bj.x;
+                        const float sn = (l1 == 0 && l2 == 1) ? sn01
+                                       : (l1 == 1 && l2 == 0) ? sn10
+                                       : (l1 == 1 && l2 == 1) ? sn11 : 0.0f;
+                        e = e + sn;
+                        if (l1 + l2 == 0) e = e_stack;
+                        val = (fc > CF_HALF) ? e + fc : CF_NEG;
+                    }
+                    v[u] = val;
                 }
-                if (bits == 0u) break;
-                const int k = 31 - __clz(bits);
-                bits &= ~(1u << k);
-                const int q = qlo + k;
-                const int l1 = p - i, l2 = j - q;
-                const int dd = s - l1 - l2 - 2;
-                const float fc = RING ? ring[(dd % CF_RING) * RS + p + 1] : fc_t[cf_dbase(L, dd) + p + 1];
-                const int qi = S.qinfo[q];
-                float e = S.cs[l1 * 31 + l2] + S.bp[sp1 * 5 + (qi >> 16)];
-                e = e + jB;
-                e = e + S.JB[(qi & 0xffff) + rowJ];
-                const float sn = (l1 == 0 && l2 == 1) ? sn01 : (l1 == 1 && l2 == 0) ? sn10 : (l1 == 1 && l2 == 1) ? sn11 : 0.0f;
-                e = e + sn;
-                if (l1 + l2 == 0) e = e_stack;
-                const float v = (fc > CF_HALF) ? e + fc : CF_NEG;
-                acc = cf_lse(acc, v, C);
+                more = (bits != 0u) || (p < pend);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc = cf_lse(acc, v[u], C);
             }
             const float fm2 = fm2d[i];
             if (fm2 > CF_HALF) {
@@ -284,7 +388,7 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
             if (RING) ring[slot * RS + i] = acc;
             fcx_t[cf_xbase(j + 1) + i - 1] = acc;
         }
-        __syncthreads();
+        cf_team_sync<PACK>();
     }
 }
 
@@ -329,6 +433,8 @@ __global__ void __launch_bounds__(128) cf_exterior_kernel(const CfSeq* __restric
     const float* __restrict__ fcx = FCX + sd.xoff;
     const float eu = P->t[CF_O_EU], ep = P->t[CF_O_EP];
     auto forced = [&](int q) -> bool { return fo != nullptr && fo[q] != 0; };
+    // k-loops run 8 terms per step: the loads of a step are issued together
+    // and the terms are then folded (F5) or scattered (F5o) in k order.
     if (!outside) {
         float* __restrict__ f5 = F5 + sd.coff;
         f5[0] = 0.0f;
@@ -338,21 +444,40 @@ __global__ void __launch_bounds__(128) cf_exterior_kernel(const CfSeq* __restric
             const int sj1 = s[j + 1];
             if (!forced(j) && sj != 4) {
                 const float* __restrict__ col = fcx + cf_xbase(j);
-                for (int k = 0; k < j; ++k) {
-                    const int sk1 = s[k + 1];
-                    if (cf_canon(sk1, sj) && !forced(k + 1)) {
-                        const float fc = col[k];
-                        const float f5k = f5[k];
-                        if (fc > CF_HALF && f5k > CF_HALF) {
-                            float jA = hc[cf_i2(sj, sk1)];
-                            if (j < L) jA = jA + dl[cf_i3(sj, sk1, sj1)];
-                            if (k > 0) jA = jA + dr[cf_i3(sj, sk1, s[k])];
-                            float v = f5k + fc;
-                            v = v + ep;
-                            v = v + bp[cf_i2(sk1, sj)];
-                            v = v + jA;
-                            acc = cf_lse(acc, v, C);
+                for (int k0 = 0; k0 < j; k0 += 8) {
+                    float fcv[8], f5v[8];
+                    int sk1v[8], skv[8];
+                    bool ok[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int k = k0 + u;
+                        ok[u] = false;
+                        fcv[u] = CF_NEG;
+                        f5v[u] = CF_NEG;
+                        sk1v[u] = 4;
+                        skv[u] = 4;
+                        if (k < j) {
+                            sk1v[u] = s[k + 1];
+                            skv[u] = s[k];
+                            ok[u] = cf_canon(sk1v[u], sj) && !forced(k + 1);
+                            fcv[u] = col[k];
+                            f5v[u] = f5[k];
                         }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int k = k0 + u;
+                        float v = CF_NEG;
+                        if (ok[u] && fcv[u] > CF_HALF && f5v[u] > CF_HALF) {
+                            float jA = hc[cf_i2(sj, sk1v[u])];
+                            if (j < L) jA = jA + dl[cf_i3(sj, sk1v[u], sj1)];
+                            if (k > 0) jA = jA + dr[cf_i3(sj, sk1v[u], skv[u])];
+                            v = f5v[u] + fcv[u];
+                            v = v + ep;
+                            v = v + bp[cf_i2(sk1v[u], sj)];
+                            v = v + jA;
+                        }
+                        acc = cf_lse(acc, v, C);
                     }
                 }
             }
@@ -371,18 +496,37 @@ __global__ void __launch_bounds__(128) cf_exterior_kernel(const CfSeq* __restric
             const int sj1 = s[j + 1];
             if (forced(j) || sj == 4) continue;
             const float* __restrict__ col = fcx + cf_xbase(j);
-            for (int k = 0; k < j; ++k) {
-                const int sk1 = s[k + 1];
-                if (cf_canon(sk1, sj) && !forced(k + 1)) {
-                    const float fc = col[k];
-                    if (fc > CF_HALF) {
-                        float jA = hc[cf_i2(sj, sk1)];
-                        if (j < L) jA = jA + dl[cf_i3(sj, sk1, sj1)];
-                        if (k > 0) jA = jA + dr[cf_i3(sj, sk1, s[k])];
+            for (int k0 = 0; k0 < j; k0 += 8) {
+                float fcv[8], f5ov[8];
+                int sk1v[8], skv[8];
+                bool ok[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int k = k0 + u;
+                    ok[u] = false;
+                    fcv[u] = CF_NEG;
+                    f5ov[u] = CF_NEG;
+                    sk1v[u] = 4;
+                    skv[u] = 4;
+                    if (k < j) {
+                        sk1v[u] = s[k + 1];
+                        skv[u] = s[k];
+                        ok[u] = cf_canon(sk1v[u], sj) && !forced(k + 1);
+                        fcv[u] = col[k];
+                        f5ov[u] = f5o[k];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int k = k0 + u;
+                    if (ok[u] && fcv[u] > CF_HALF) {
+                        float jA = hc[cf_i2(sj, sk1v[u])];
+                        if (j < L) jA = jA + dl[cf_i3(sj, sk1v[u], sj1)];
+                        if (k > 0) jA = jA + dr[cf_i3(sj, sk1v[u], skv[u])];
                         float temp = f5oj + ep;
-                        temp = temp + bp[cf_i2(sk1, sj)];
+                        temp = temp + bp[cf_i2(sk1v[u], sj)];
                         temp = temp + jA;
-                        f5o[k] = cf_lse(f5o[k], temp + fc, C);
+                        f5o[k] = cf_lse(f5ov[u], temp + fcv[u], C);
                     }
                 }
             }
@@ -398,21 +542,26 @@ __global__ void __launch_bounds__(128) cf_exterior_kernel(const CfSeq* __restric
 // POST of the pair (a, b+1) is gathered in the same step as FCo[a][b]: its
 // single-branch sources are exactly FCo's, visited in reverse order.
 // POSTF is indexed by FC cell: POSTF[tri(a, b)] = P(a pairs b+1).
-template <bool RING>
+template <bool RING, bool PACK>
 __global__ void __launch_bounds__(256) cf_outside_kernel(
-    const CfSeq* __restrict__ seqs, const int* __restrict__ order, const uint8_t* __restrict__ codes,
+    const CfSeq* __restrict__ seqs, const int* __restrict__ order, int nseq, const uint8_t* __restrict__ codes,
     const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
     const float* __restrict__ FM, const float* __restrict__ FM1, const float* __restrict__ F5,
     const float* __restrict__ F5o, float* __restrict__ FCo, float* __restrict__ FMo, float* __restrict__ FM1o,
     float* __restrict__ FM2o, float* __restrict__ POSTF, int Lmax) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const CfLds plan = cf_lds_plan(Lmax, RING, 4);
-    const CfShared S = cf_carve(smem, plan);
-    const CfSeq sd = seqs[order[blockIdx.x]];
+    const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
+    const CfTeam tm = cf_team<PACK>(G);
+    const CfLds plan = cf_lds_plan(Lmax, RING, 4, G);
+    const CfShared S = cf_carve(smem, plan, tm.team);
+    cf_stage_params(S, P);
+    __syncthreads();
+    if (tm.li >= nseq) return;
+    const CfSeq sd = seqs[order[tm.li]];
     const int L = sd.L;
-    const int tid = threadIdx.x, NT = blockDim.x;
+    const int tid = tm.tid, NT = tm.NT;
     const int lane = tid & (WAVE - 1);
-    cf_stage(S, P, codes, masks, sd);
+    cf_stage_seq<PACK>(S, codes, masks, sd, tid, NT);
 
     const float mb = P->t[CF_O_MB], mu = P->t[CF_O_MU], mp = P->t[CF_O_MP], ep = P->t[CF_O_EP];
     const float* __restrict__ T = P->t;
@@ -438,7 +587,7 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
         const int64_t db = cf_dbase(L, s);
         const int slot = s % CF_RING;
         // ---- phase 1: FMo for every multiloop cell; collect pairable cells
-        for (int i0 = tid - lane; i0 < ncell; i0 += NT) {
+        for (int i0 = __builtin_amdgcn_readfirstlane(tid - lane); i0 < ncell; i0 += NT) {
             const int a = i0 + lane;
             const int b = a + s;
             const bool live = a < ncell;
@@ -448,7 +597,24 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             // (i) FM2o(i', b) + FM1[i'][a] for i' = 1 .. a-2 ascending, i.e. u = a - i' descending.
             // Lanes of a wave step u together so both reads stay contiguous.
             const int amax = min(i0 + WAVE - 1, ncell - 1);
-            for (int u = amax - 1; u >= 2; --u) {
+            int u = amax - 1;
+            for (; u - 7 >= 2; u -= 8) {
+                float x[8], y[8];
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    const int uu = u - w;
+                    x[w] = CF_NEG;
+                    y[w] = CF_NEG;
+                    if (dom && uu <= a - 1) {
+                        x[w] = fm2o_t[cf_dbase(L, s + uu) + a - uu];
+                        y[w] = fm1_t[cf_dbase(L, uu) + a - uu];
+                    }
+                }
+#pragma unroll
+                for (int w = 0; w < 8; ++w)
+                    fmo = cf_lse(fmo, (x[w] > CF_HALF && y[w] > CF_HALF) ? x[w] + y[w] : CF_NEG, C);
+            }
+            for (; u >= 2; --u) {
                 if (dom && u <= a - 1) {
                     const int ip = a - u;
                     const float x = fm2o_t[cf_dbase(L, s + u) + ip];
@@ -474,7 +640,7 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
                 }
             }
         }
-        __syncthreads();
+        cf_team_sync<PACK>();
         // ---- phase 2: FCo and POST for pairable cells
         const int nal = S.cnt[s & 1];
         if (tid == 0) S.cnt[(s + 1) & 1] = 0;
@@ -615,9 +781,9 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             }
             post_t[db + a] = post;
         }
-        __syncthreads();
+        cf_team_sync<PACK>();
         // ---- phase 3: FM2o and FM1o for every multiloop cell
-        for (int i0 = tid - lane; i0 < ncell; i0 += NT) {
+        for (int i0 = __builtin_amdgcn_readfirstlane(tid - lane); i0 < ncell; i0 += NT) {
             const int a = i0 + lane;
             const int b = a + s;
             const bool live = a < ncell;
@@ -646,7 +812,24 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             }
             // (ii) FM2o(a, b+v) + FM[b][b+v] for v = L-1-b .. 2 descending (lanes step v together)
             const int vmax = L - 1 - (i0 + s);
-            for (int v = vmax; v >= 2; --v) {
+            int v = vmax;
+            for (; v - 7 >= 2; v -= 8) {
+                float x[8], y[8];
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    const int vv = v - w;
+                    x[w] = CF_NEG;
+                    y[w] = CF_NEG;
+                    if (dom && vv <= L - 1 - b) {
+                        x[w] = fm2o_t[cf_dbase(L, s + vv) + a];
+                        y[w] = fm_t[cf_dbase(L, vv) + b];
+                    }
+                }
+#pragma unroll
+                for (int w = 0; w < 8; ++w)
+                    f1 = cf_lse(f1, (x[w] > CF_HALF && y[w] > CF_HALF) ? x[w] + y[w] : CF_NEG, C);
+            }
+            for (; v >= 2; --v) {
                 if (dom && v <= L - 1 - b) {
                     const float x = fm2o_t[cf_dbase(L, s + v) + a];
                     const float y = fm_t[cf_dbase(L, v) + b];
@@ -662,7 +845,7 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
                 fm1o_t[db + a] = f1;
             }
         }
-        __syncthreads();
+        cf_team_sync<PACK>();
     }
 }
 
@@ -727,7 +910,24 @@ __global__ void __launch_bounds__(256) cf_mea_kernel(const CfSeq* __restrict__ s
                     }
                     int64_t ia = cf_dbase(L, 1) + i;
                     int64_t ib = cf_dbase(L, s - 1) + i + 1;
-                    for (int d = 1; d < s; ++d) {
+                    int d = 1;
+                    for (; d + 7 < s; d += 8) {  // 8 splits' loads in flight, compared in k order
+                        float x[8], y[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            x[u] = sc[ia];
+                            y[u] = sc[ib];
+                            ia += L + 1 - (d + u);
+                            ib -= L + 1 - (s - (d + u) - 1);
+                            ib += 1;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const float w = x[u] + y[u];
+                            if (w > best) { best = w; t = i + d + u + 4; }
+                        }
+                    }
+                    for (; d < s; ++d) {
                         v = sc[ia] + sc[ib];
                         if (v > best) { best = v; t = i + d + 4; }
                         ia += L + 1 - d;
@@ -830,7 +1030,22 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
             if (inner && s >= 4) {
                 int64_t ia = cf_dbase(L, 2) + i;
                 int64_t ib = cf_dbase(L, s - 2) + i + 2;
-                for (int t = 2; t <= s - 2; ++t) {
+                int t = 2;
+                for (; t + 7 <= s - 2; t += 8) {
+                    double x[8], y[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        x[u] = fm1_t[ia];
+                        y[u] = fm_t[ib];
+                        ia += L + 1 - (t + u);
+                        ib -= L + 1 - (s - (t + u) - 1);
+                        ib += 1;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (x[u] > CF_HALF64 && y[u] > CF_HALF64) cf_acc<LSE>(fm2, x[u] + y[u]);
+                }
+                for (; t <= s - 2; ++t) {
                     const double x = fm1_t[ia], y = fm_t[ib];
                     if (x > CF_HALF64 && y > CF_HALF64) cf_acc<LSE>(fm2, x + y);
                     ia += L + 1 - t;
@@ -1517,18 +1732,24 @@ __global__ void __launch_bounds__(256) cf_grad_kernel(
 }
 
 // ------------------------------------------------------------ explicit inits
-template __global__ void cf_inside_kernel<true>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
-                                                const CfParams<float>*, float*, float*, float*, float*, float*, int);
-template __global__ void cf_inside_kernel<false>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
-                                                 const CfParams<float>*, float*, float*, float*, float*, float*, int);
-template __global__ void cf_outside_kernel<true>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
-                                                 const CfParams<float>*, const float*, const float*, const float*,
-                                                 const float*, const float*, float*, float*, float*, float*, float*,
-                                                 int);
-template __global__ void cf_outside_kernel<false>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
-                                                  const CfParams<float>*, const float*, const float*, const float*,
-                                                  const float*, const float*, float*, float*, float*, float*, float*,
-                                                  int);
+template __global__ void cf_inside_kernel<false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                        const CfParams<float>*, float*, float*, float*, float*, float*,
+                                                        int);
+template __global__ void cf_inside_kernel<false, true>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                       const CfParams<float>*, float*, float*, float*, float*, float*,
+                                                       int);
+template __global__ void cf_outside_kernel<true, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                        const CfParams<float>*, const float*, const float*, const float*,
+                                                        const float*, const float*, float*, float*, float*, float*,
+                                                        float*, int);
+template __global__ void cf_outside_kernel<false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                         const CfParams<float>*, const float*, const float*,
+                                                         const float*, const float*, const float*, float*, float*,
+                                                         float*, float*, float*, int);
+template __global__ void cf_outside_kernel<false, true>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                        const CfParams<float>*, const float*, const float*, const float*,
+                                                        const float*, const float*, float*, float*, float*, float*,
+                                                        float*, int);
 template __global__ void cf_viterbi_kernel<true>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
                                                  const CfParams<double>*, double*, double*, double*, double*, uint8_t*,
                                                  const int64_t*, int);

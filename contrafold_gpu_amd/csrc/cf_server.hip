@@ -3,9 +3,10 @@
 //
 // Requests wait in a FIFO.  Whenever one of the server's streams is idle, the
 // scheduler takes the head request and every later pending request of the
-// same kind (MEA / sample / bpp / logZ, same gamma and sample count), up to
-// max_batch, as long as their device table bytes (cf_fold_bytes, O(L^2)) fit
-// the HBM budget next to the batches already in flight.  The batch is packed
+// same kind (MEA / sample / bpp / logZ, same gamma and sample count) and the
+// same length class, up to max_batch, as long as their device table bytes
+// (cf_fold_bytes, O(L^2)) fit the HBM budget next to the batches already in
+// flight.  The batch is packed
 // and uploaded asynchronously and its kernels are enqueued on that stream, so
 // batches admitted while others run overlap with them on the GPU.  Completion
 // is detected with events; results are copied out and the memory returns to
@@ -49,6 +50,12 @@ struct Slot {
     unsigned kind = 0;
     int64_t bytes = 0;
 };
+
+// Length classes: a batch retires when its slowest fold finishes, so a batch
+// only takes requests of one class and short folds never wait on long ones.
+int length_class(int L) {
+    return L <= 96 ? 0 : L <= 192 ? 1 : L <= 320 ? 2 : L <= 512 ? 3 : L <= 768 ? 4 : 5;
+}
 
 unsigned kind_flags(unsigned kind) {
     switch (kind) {
@@ -128,6 +135,7 @@ int launch(cf_server* s, Slot& sl) {
     const unsigned kind = head.kind;
     const float gamma = head.gamma;
     const int nsamp = head.n_samples;
+    const int lclass = length_class((int)head.seq.size());
     if (head.bytes > s->budget) {  // can never fit: fail it
         Req& r = s->reqs[head.id];
         r.state = 3;
@@ -142,7 +150,8 @@ int launch(cf_server* s, Slot& sl) {
     bool blocked = false;
     for (int64_t id : s->pending) {
         Req& r = s->reqs[id];
-        const bool same = r.kind == kind && r.gamma == gamma && r.n_samples == nsamp;
+        const bool same = r.kind == kind && r.gamma == gamma && r.n_samples == nsamp &&
+                          length_class((int)r.seq.size()) == lclass;
         if (!blocked && same && (int)pick.size() < s->max_batch && s->in_use + bytes + r.bytes <= s->budget) {
             pick.push_back(id);
             bytes += r.bytes;

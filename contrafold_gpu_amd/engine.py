@@ -198,12 +198,13 @@ class Batch:
               self.engine.ctx, "cf_batch_bpp")
         return out
 
-    def _split(self, raw, per=1):
+    def _split(self, raw, per=None):
+        """One string per sequence (per=None) or a list of ``per`` strings."""
         s = raw.decode("ascii")
         out = []
         for k in range(self.n):
-            a, L = int(self.offs[k]) * per, int(self.lens[k])
-            if per == 1:
+            a, L = int(self.offs[k]) * (per or 1), int(self.lens[k])
+            if per is None:
                 out.append(s[a:a + L])
             else:
                 out.append([s[a + t * L:a + (t + 1) * L] for t in range(per)])

@@ -32,7 +32,7 @@ def request_key(seq, mask=None, seed=0):
 
 
 class Server:
-    def __init__(self, engine, hbm_budget=64 << 30, max_batch=512, n_streams=4):
+    def __init__(self, engine, hbm_budget=64 << 30, max_batch=512, n_streams=8):
         self.engine = engine
         self.lib = engine.lib
         h = _c.c_void_p()

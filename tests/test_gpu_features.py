@@ -298,3 +298,21 @@ def test_uneven_lengths_buckets_and_n_bases(engine, P, oracle):
             pair, _ = oracle.mea_decode32(POST, s, g)
             assert got[g_i][k] == oracle.dot_bracket(pair, len(s))
         assert vit[k] == oracle.mfe(s, P)
+
+
+def test_cli_contrafold_formats(P, tmp_path, capsys):
+    from contrafold_gpu_amd import cli
+    import contrafold_gpu_amd as cf
+    s = "GCGCGCAAAAGCGCGCAAAAGCGC"
+    cli.main([s, "--posteriors", "1e-4"])
+    text = capsys.readouterr().out
+    M = cli.parse_posteriors(text, len(s))
+    ref = cf.bpp(s, P)
+    keep = ref >= 1e-4
+    assert np.allclose(M[keep], ref[keep], rtol=1e-5) and not M[~keep].any()
+    cli.main([s, "--partition"])
+    line = capsys.readouterr().out.strip()
+    assert line.startswith('Log partition coefficient for "0":')
+    assert abs(float(line.split()[-1]) - cf.logZ_batch([s], P)[0]) < 1e-5
+    cli.main([s, "--parens", "-o", str(tmp_path / "p.txt")])
+    assert (tmp_path / "p.txt").read_text().splitlines() == [">0", s, cf.mea(s, P)]

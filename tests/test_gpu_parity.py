@@ -183,11 +183,21 @@ def test_dropin_api_known_answers(P):
         cf.logZ_batch(["GGGAAACCC"], P, forced_list=[[0, 1]])
 
 
-def test_long_sequence_parity(engine, P, oracle):
-    """One 420-nt sequence (the ring-buffer path with many diagonals)."""
+def _wide_engine(P):
+    from contrafold_gpu_amd import Engine
+    eng = Engine()
+    eng.set_params(P)
+    eng.set_tuning([1 << 30], [256])
+    return eng
+
+
+def test_long_sequence_parity(P, oracle):
+    """One 420-nt sequence on a 256-thread workgroup (the outside sweep keeps
+    its FCo window in an LDS ring)."""
     from contrafold_gpu_amd import CF_MEA
     rng = np.random.default_rng(99)
     s = rand_seq(rng, 420)
+    engine = _wide_engine(P)
     with engine.batch([s]) as b:
         b.run(CF_MEA, gammas=[6.0])
         z32, POST = oracle.posterior32(s, P)
@@ -195,14 +205,18 @@ def test_long_sequence_parity(engine, P, oracle):
         got = b.bpp(0)
         assert (got.view(np.uint32) == POST[1:421, 1:421].view(np.uint32)).all()
         assert b.mea(0)[0] == oracle.mea(s, P, 6.0)
+    engine.close()
 
 
-def test_beyond_lds_ring_parity(engine, P, oracle):
-    """L = 1100: the FC/FCo ring no longer fits in LDS, so the kernels read the
-    window from the global tables (the long-sequence path)."""
+@pytest.mark.parametrize("width", [256, 64])
+def test_beyond_lds_ring_parity(P, oracle, width):
+    """L = 1100 on a 256-thread workgroup (the FCo ring no longer fits in LDS,
+    so the window is read from the global tables) and packed (one wave)."""
     from contrafold_gpu_amd import CF_MEA
     rng = np.random.default_rng(1100)
     s = rand_seq(rng, 1100)
+    engine = _wide_engine(P)
+    engine.set_tuning([1 << 30], [width])
     with engine.batch([s]) as b:
         b.run(CF_MEA, gammas=[6.0])
         z32, POST = oracle.posterior32(s, P)
@@ -211,3 +225,33 @@ def test_beyond_lds_ring_parity(engine, P, oracle):
         assert (got.view(np.uint32) == POST[1:1101, 1:1101].view(np.uint32)).all()
         pair, _ = oracle.mea_decode32(POST, s, 6.0)
         assert b.mea(0)[0] == oracle.dot_bracket(pair, 1100)
+    engine.close()
+
+
+def test_packed_layout_parity(P, oracle):
+    """One wave per sequence (packed workgroups; the single-branch windows are
+    read from the global tables): logZ, every posterior and the MEA structure
+    bit-exact against the oracle, lengths 300 down to 1, with masks."""
+    from contrafold_gpu_amd import CF_MEA, Engine
+    rng = np.random.default_rng(64)
+    seqs, masks = [], []
+    for L in (300, 257, 130, 65, 64, 17, 5, 1):
+        seqs.append(rand_seq(rng, L))
+        masks.append((rng.random(L) < 0.1).astype(np.int8) if L % 2 else None)
+    eng = Engine()
+    eng.set_params(P)
+    eng.set_tuning([1 << 30], [64])
+    try:
+        with eng.batch(seqs, masks) as b:
+            b.run(CF_MEA, gammas=[6.0])
+            z = b.logz()
+            got = b.mea(0)
+            for k, (s, m) in enumerate(zip(seqs, masks)):
+                z32, POST = oracle.posterior32(s, P, m)
+                L = len(s)
+                assert z[k].view(np.uint32) == np.float32(z32).view(np.uint32), k
+                assert (b.bpp(k).view(np.uint32) == POST[1:L + 1, 1:L + 1].view(np.uint32)).all(), k
+                pair, _ = oracle.mea_decode32(POST, s, 6.0, m)
+                assert got[k] == oracle.dot_bracket(pair, L), k
+    finally:
+        eng.close()
