@@ -221,7 +221,7 @@ def main():
     # algorithmic work (exact counts on a sample, scaled to the batch)
     wc = workcount.batch_counts(seqs, masks, sample=24)
     flops = {"inside": 9.0 * wc["inside"], "exterior": 9.0 * wc["exterior"],
-             "outside": 9.0 * (wc["outside"] + wc["posterior"]), "mea": 2.0 * wc["mea"]}
+             "outside": 9.0 * wc["outside"], "post": 9.0 * wc["posterior"], "mea": 2.0 * wc["mea"]}
     dominant = max(kavg, key=kavg.get)
     roof = None
     if dominant in flops and kavg[dominant] > 0:

@@ -142,7 +142,7 @@ int launch_outside(cf_batch* b) {
             if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
             hipLaunchKernelGGL((cf_outside_kernel<false, true>), dim3((nl + kPack - 1) / kPack), dim3(64 * kPack),
                                plan.total, b->stream, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks,
-                               c->dP32, b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST,
+                               c->dP32, b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o,
                                bk.Lmax);
             CF_HIP(c, hipGetLastError(), "outside launch");
             continue;
@@ -154,12 +154,30 @@ int launch_outside(cf_batch* b) {
         if (ring)
             hipLaunchKernelGGL((cf_outside_kernel<true, false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
                                b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
-                               b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST, bk.Lmax);
+                               b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
         else
             hipLaunchKernelGGL((cf_outside_kernel<false, false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
                                b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
-                               b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, b->POST, bk.Lmax);
+                               b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
         CF_HIP(c, hipGetLastError(), "outside launch");
+    }
+    return CF_OK;
+}
+
+int launch_post(cf_batch* b) {
+    cf_ctx* c = b->ctx;
+    for (const auto& bk : b->buckets) {
+        const int nl = bk.end - bk.begin;
+        if (nl <= 0) continue;
+        // packed buckets: kPack sequences per workgroup; wide buckets: the
+        // workgroup's kPack waves split one sequence's diagonals
+        const bool pack = bk.threads <= 64;
+        const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, kPack);
+        if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the posterior kernel's LDS plan");
+        hipLaunchKernelGGL(cf_post_kernel, dim3(pack ? (nl + kPack - 1) / kPack : nl), dim3(64 * kPack), plan.total,
+                           b->stream, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC,
+                           b->F5, b->F5o, b->FCo, b->FM1o, b->POST, bk.Lmax, pack ? 1 : kPack);
+        CF_HIP(c, hipGetLastError(), "posterior launch");
     }
     return CF_OK;
 }
@@ -327,25 +345,29 @@ int enqueue_run(cf_batch* b, unsigned need, uint64_t seed, bool timing) {
         if ((rc = launch_outside(b))) return rc;
     }
     mark(3);
+    if (need & CF_BPP) {
+        if ((rc = launch_post(b))) return rc;
+    }
+    mark(4);
     if (need & CF_MEA) {
         if ((rc = launch_mea(b))) return rc;
     }
-    mark(4);
+    mark(5);
     if (need & CF_MFE) {
         if ((rc = launch_viterbi(b, false))) return rc;
     }
     if (need & CF_LOGZ64) {
         if ((rc = launch_viterbi(b, true))) return rc;
     }
-    mark(5);
+    mark(6);
     if (need & CF_SAMPLE) {
         if ((rc = launch_sample(b, seed))) return rc;
     }
-    mark(6);
+    mark(7);
     if (need & CF_GRAD) {
         if ((rc = launch_grad(b))) return rc;
     }
-    mark(7);
+    mark(8);
     return CF_OK;
 }
 
@@ -615,7 +637,7 @@ int cf_create(int device, cf_ctx** out) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0) c->n_cu = v;
     const void* dyn[] = {(const void*)cf_inside_kernel<false, false>, (const void*)cf_inside_kernel<false, true>,
                          (const void*)cf_outside_kernel<true, false>, (const void*)cf_outside_kernel<false, false>,
-                         (const void*)cf_outside_kernel<false, true>,
+                         (const void*)cf_outside_kernel<false, true>, (const void*)cf_post_kernel,
                          (const void*)cf_mea_kernel, (const void*)cf_viterbi_kernel<false>,
                          (const void*)cf_viterbi_kernel<true>};
     for (const void* f : dyn) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);

@@ -12,9 +12,9 @@
 
 namespace cfi {
 
-static const char* kKernelNames[] = {"inside", "exterior", "outside", "mea", "viterbi", "sample", "grad"};
-constexpr int kNumKernels = 7;
-enum { K_INSIDE = 0, K_EXTERIOR, K_OUTSIDE, K_MEA, K_VITERBI, K_SAMPLE, K_GRAD };
+static const char* kKernelNames[] = {"inside", "exterior", "outside", "post", "mea", "viterbi", "sample", "grad"};
+constexpr int kNumKernels = 8;
+enum { K_INSIDE = 0, K_EXTERIOR, K_OUTSIDE, K_POST, K_MEA, K_VITERBI, K_SAMPLE, K_GRAD };
 
 // Caching device allocator: blocks are rounded to 2 MiB and kept on a free
 // list after release, so steady-state batches do not call hipMalloc.
@@ -117,7 +117,7 @@ struct cf_batch {
     double* grad = nullptr;
     unsigned done = 0;
     uint64_t params_version = 0;
-    hipEvent_t ev[cfi::kNumKernels + 3] = {};  // stage marks 0..7, run start, run end
+    hipEvent_t ev[cfi::kNumKernels + 3] = {};  // stage marks 0..8, run start, run end
     float last_ms = 0.0f;
     float kms[cfi::kNumKernels] = {};
     hipGraphExec_t gexec = nullptr;

@@ -34,7 +34,7 @@ struct CfLds {
     // shared area (offsets from the start of LDS)
     int o_lse, o_exp, o_cs, o_jb, o_bp, o_bj, shared;
     // team area (offsets from the team's base)
-    int o_qinfo, o_s8, o_fo8, o_rs8, o_pmask, o_dbuf, o_alist, o_cnt, o_ring, team;
+    int o_qinfo, o_dtab, o_qo, o_s8, o_fo8, o_rs8, o_pmask, o_dbuf, o_alist, o_cnt, o_ring, team;
     int G, total;
 };
 __host__ __device__ inline int cf_align(int x, int a) { return (x + a - 1) / a * a; }
@@ -52,13 +52,16 @@ __host__ __device__ inline CfLds cf_lds_plan(int Lmax, bool ring, int dbuf_elem,
     p.shared = cf_align(o, 16);
     o = 0;
     p.o_qinfo = o; o += (Lmax + 3) * 4;
+    p.o_dtab = o; o += (Lmax + 1) * 4;
+    p.o_qo = o; o += (Lmax + 3) * 2;
+    o = cf_align(o, 4);
     p.o_s8 = o; o += Lmax + 3;
     p.o_fo8 = o; o += Lmax + 3;
     p.o_rs8 = o; o += Lmax + 3;
     o = cf_align(o, 16);
     p.o_pmask = o; o += 5 * p.W * 4;
     o = cf_align(o, 16);
-    p.o_dbuf = o; o += (Lmax + 1) * dbuf_elem;
+    p.o_dbuf = o; o += (Lmax + 1 > 128 ? Lmax + 1 : 128) * dbuf_elem;  // also the posterior queue
     o = cf_align(o, 16);
     p.o_alist = o; o += (Lmax + 1) * 2;
     o = cf_align(o, 16);
@@ -79,6 +82,8 @@ struct CfShared {
     float* bp;
     float2* BJ;  // (JB[q][p+1][q+1][p], bp[p+1][q]) for one 8-byte read per candidate
     int* qinfo;
+    int* dtab;  // dtab[d] = dbase(L, d), the offset of diagonal d in a triangular table
+    int16_t* qo;  // qo[q] = 25*s[q+1] + s[q]: the (q, q+1) part of an outer-pair junction index
     int8_t* s8;
     int8_t* fo8;
     int8_t* rs8;
@@ -100,6 +105,8 @@ __device__ inline CfShared cf_carve(unsigned char* smem, const CfLds& p, int tea
     S.BJ = (float2*)(smem + p.o_bj);
     unsigned char* t = smem + p.shared + team * p.team;
     S.qinfo = (int*)(t + p.o_qinfo);
+    S.dtab = (int*)(t + p.o_dtab);
+    S.qo = (int16_t*)(t + p.o_qo);
     S.s8 = (int8_t*)(t + p.o_s8);
     S.fo8 = (int8_t*)(t + p.o_fo8);
     S.rs8 = (int8_t*)(t + p.o_rs8);
@@ -155,10 +162,12 @@ __device__ void cf_stage_seq(const CfShared& S, const uint8_t* __restrict__ code
     }
     if (tid == 0) { S.s8[L + 2] = 4; S.fo8[L + 2] = 0; S.cnt[0] = 0; S.cnt[1] = 0; }
     cf_team_sync<PACK>();
+    for (int d = tid; d <= L; d += NT) S.dtab[d] = (int)cf_dbase(L, d);
     for (int q = tid; q < L + 2; q += NT) {
         const int c = S.s8[q];
         S.rs8[q] = (S.fo8[q] || c == 4) ? 4 : c;
         S.qinfo[q] = (c << 16) | (c * 125 + S.s8[q + 1] * 5);
+        S.qo[q] = (int16_t)(S.s8[q + 1] * 25 + c);
     }
     for (int w = tid; w < S.W; w += NT) {
         uint32_t m[5] = {0u, 0u, 0u, 0u, 0u};
@@ -355,7 +364,7 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
                         const int q = qlo + k;
                         const int l1 = p - i, l2 = j - q;
                         const int dd = s - l1 - l2 - 2;
-                        const float fc = RING ? ring[(dd % CF_RING) * RS + p + 1] : fc_t[cf_dbase(L, dd) + p + 1];
+                        const float fc = RING ? ring[(dd % CF_RING) * RS + p + 1] : fc_t[S.dtab[dd] + p + 1];
                         const int qi = S.qinfo[q];
                         const float2 bj = S.BJ[(qi & 0xffff) + rowJ];
                         float e = S.cs[l1 * 31 + l2] + bj.y;
@@ -548,7 +557,7 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
     const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
     const float* __restrict__ FM, const float* __restrict__ FM1, const float* __restrict__ F5,
     const float* __restrict__ F5o, float* __restrict__ FCo, float* __restrict__ FMo, float* __restrict__ FM1o,
-    float* __restrict__ FM2o, float* __restrict__ POSTF, int Lmax) {
+    float* __restrict__ FM2o, int Lmax) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
     const CfTeam tm = cf_team<PACK>(G);
@@ -577,7 +586,6 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
     float* __restrict__ fmo_t = FMo + sd.toff;
     float* __restrict__ fm1o_t = FM1o + sd.toff;
     float* __restrict__ fm2o_t = FM2o + sd.toff;
-    float* __restrict__ post_t = POSTF + sd.toff;
     const float4* __restrict__ C = S.lse;
     const float4* __restrict__ E = S.ex;
     const float Z = f5[L];
@@ -598,29 +606,38 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             // Lanes of a wave step u together so both reads stay contiguous.
             const int amax = min(i0 + WAVE - 1, ncell - 1);
             int u = amax - 1;
+            // wave-uniform offsets of diagonals s + u (FM2o) and u (FM1), stepped
+            // down with u; lanes outside the domain read cell 0 of the diagonal
+            // and fold NEG, which leaves the sum unchanged
+            int ox = (int)cf_dbase(L, s + u), oy = (int)cf_dbase(L, u);
             for (; u - 7 >= 2; u -= 8) {
                 float x[8], y[8];
+                bool g[8];
 #pragma unroll
                 for (int w = 0; w < 8; ++w) {
                     const int uu = u - w;
-                    x[w] = CF_NEG;
-                    y[w] = CF_NEG;
-                    if (dom && uu <= a - 1) {
-                        x[w] = fm2o_t[cf_dbase(L, s + uu) + a - uu];
-                        y[w] = fm1_t[cf_dbase(L, uu) + a - uu];
-                    }
+                    g[w] = dom & (uu <= a - 1);
+                    const int off = g[w] ? a - uu : 0;
+                    x[w] = fm2o_t[ox + off];
+                    y[w] = fm1_t[oy + off];
+                    ox -= L + 2 - (s + uu);
+                    oy -= L + 2 - uu;
                 }
 #pragma unroll
-                for (int w = 0; w < 8; ++w)
-                    fmo = cf_lse(fmo, (x[w] > CF_HALF && y[w] > CF_HALF) ? x[w] + y[w] : CF_NEG, C);
+                for (int w = 0; w < 8; ++w) {
+                    const bool ok = g[w] & (x[w] > CF_HALF) & (y[w] > CF_HALF);
+                    fmo = cf_lse(fmo, ok ? x[w] + y[w] : CF_NEG, C);
+                }
             }
             for (; u >= 2; --u) {
-                if (dom && u <= a - 1) {
-                    const int ip = a - u;
-                    const float x = fm2o_t[cf_dbase(L, s + u) + ip];
-                    const float y = fm1_t[cf_dbase(L, u) + ip];
-                    if (x > CF_HALF && y > CF_HALF) fmo = cf_lse(fmo, x + y, C);
-                }
+                const bool g = dom & (u <= a - 1);
+                const int off = g ? a - u : 0;
+                const float x = fm2o_t[ox + off];
+                const float y = fm1_t[oy + off];
+                ox -= L + 2 - (s + u);
+                oy -= L + 2 - u;
+                const bool ok = g & (x > CF_HALF) & (y > CF_HALF);
+                fmo = cf_lse(fmo, ok ? x + y : CF_NEG, C);
             }
             // (ii) FMo[a][b+1] + mu
             if (dom && b + 1 < L) {
@@ -636,12 +653,11 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
                 } else {
                     fco_t[db + a] = CF_NEG;
                     if (RING) ring[slot * RS + a] = CF_NEG;
-                    post_t[db + a] = 0.0f;
                 }
             }
         }
         cf_team_sync<PACK>();
-        // ---- phase 2: FCo and POST for pairable cells
+        // ---- phase 2: FCo for pairable cells
         const int nal = S.cnt[s & 1];
         if (tid == 0) S.cnt[(s + 1) & 1] = 0;
         for (int idx = tid; idx < nal; idx += NT) {
@@ -649,18 +665,18 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             const int b = a + s;
             const int sa = S.s8[a], sb1 = S.s8[b + 1], sam = S.s8[a - 1], sb2 = S.s8[b + 2];
             const float bpab = S.bp[cf_i2(sa, sb1)];
-            const float fcab = fc_t[db + a];
             // jA of branch (a, b+1) seen from the enclosing loop (exterior or multiloop)
             float jAx = T[CF_O_HC + cf_i2(sb1, sa)];
             if (b + 1 < L) jAx = jAx + T[CF_O_DL + cf_i3(sb1, sa, sb2)];
             if (a - 1 > 0) jAx = jAx + T[CF_O_DR + cf_i3(sb1, sa, sam)];
             const float jB2 = S.JB[cf_i4(sb1, sa, sb2, sam)];
             const float e_stack = bpab + T[CF_O_STACK + cf_i4(sam, sb2, sa, sb1)];
+            // bulge 0x1 / 1x0 and interior 1x1 extras of the three nearest outer pairs
+            const float sn01 = T[CF_O_B01 + sb2], sn10 = T[CF_O_B01 + sam], sn11 = T[CF_O_I11 + cf_i2(sam, sb2)];
             const bool fm1dom = (a - 1 > 0) && (a + 1 <= b + 1) && (b + 1 < L);
             const float fm1o_src = fm1dom ? fm1o_t[cf_dbase(L, s + 2) + a - 1] : CF_NEG;
             const int lmax = min(CF_CMAX, a - 2);
             const int qbase = b + 2;
-            // ---------- FCo[a][b]
             float acc = CF_NEG;
             const float f5a = f5[a - 1];
             if (f5a > CF_HALF) {
@@ -669,117 +685,44 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
                 temp = temp + jAx;
                 acc = temp + f5a;
             }
+            // Sources in the reference's order: outer rows i' = a-1-l1 ascending
+            // (l1 descending), partners descending; in row l1 = 0 the FM1o source
+            // of iteration (a-1, b+1) comes after every k >= 1 and before k = 0.
             for (int l1 = lmax; l1 >= 0; --l1) {
                 const int ip = a - 1 - l1;
                 uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
-                const int sip = S.s8[ip], sip1 = S.s8[ip + 1];
-                bool fm1_pending = (l1 == 0);
-                while (true) {
-                    if (fm1_pending && (bits & ~1u) == 0u) {
-                        // the FM1o source of iteration (a-1, b+1) precedes its stack term
-                        fm1_pending = false;
-                        if (fm1o_src > CF_HALF) {
-                            float v = fm1o_src + jAx;
-                            v = v + mp;
-                            v = v + bpab;
-                            acc = cf_lse(acc, v, C);
-                        }
-                    }
-                    if (bits == 0u) break;
-                    const int k = 31 - __clz(bits);
-                    bits &= ~(1u << k);
-                    const int jp = qbase + k - 1;  // outer cell (ip, jp)
-                    const int l2 = k;
-                    const int sl = (s + l1 + l2 + 2) % CF_RING;
-                    const float fco = RING ? ring[sl * RS + ip] : fco_t[cf_dbase(L, s + l1 + l2 + 2) + ip];
-                    const int sjp = S.s8[jp];
+                const int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
+                const uint32_t last = (l1 == 0) ? (bits & 1u) : 0u;
+                bits &= (l1 == 0) ? ~1u : ~0u;
+                auto term = [&](int l2) -> float {
+                    const int d = s + l1 + l2 + 2;
+                    const float fco = RING ? ring[(d % CF_RING) * RS + ip] : fco_t[S.dtab[d] + ip];
                     float e = S.cs[l1 * 31 + l2] + bpab;
-                    e = e + S.JB[cf_i4(sip, S.s8[jp + 1], sip1, sjp)];
+                    e = e + S.JB[rowJ + S.qo[qbase + l2 - 1]];
                     e = e + jB2;
-                    const float sn = (l1 == 0 && l2 == 1) ? T[CF_O_B01 + sjp]
-                                   : (l1 == 1 && l2 == 0) ? T[CF_O_B01 + sip1]
-                                   : (l1 == 1 && l2 == 1) ? T[CF_O_I11 + cf_i2(sip1, sjp)] : 0.0f;
+                    const float sn = (l1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
+                                   : (l1 == 1) ? (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f)) : 0.0f;
                     e = e + sn;
-                    if (l1 + l2 == 0) e = e_stack;
-                    const float v = (fco > CF_HALF) ? fco + e : CF_NEG;
-                    acc = cf_lse(acc, v, C);
+                    e = (l1 + l2 == 0) ? e_stack : e;
+                    return (fco > CF_HALF) ? fco + e : CF_NEG;
+                };
+                while (bits != 0u) {
+                    const int kk = 31 - __clz(bits);
+                    bits &= ~(1u << kk);
+                    acc = cf_lse(acc, term(kk), C);
+                }
+                if (l1 == 0) {
+                    if (fm1o_src > CF_HALF) {
+                        float v = fm1o_src + jAx;
+                        v = v + mp;
+                        v = v + bpab;
+                        acc = cf_lse(acc, v, C);
+                    }
+                    if (last) acc = cf_lse(acc, term(0), C);
                 }
             }
             fco_t[db + a] = acc;
             if (RING) ring[slot * RS + a] = acc;
-            // ---------- POST(a, b+1): same sources, reverse order, then exterior
-            float post = 0.0f;
-            if (fcab > CF_HALF) {
-                for (int l1 = 0; l1 <= lmax; ++l1) {
-                    const int ip = a - 1 - l1;
-                    uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
-                    const int sip = S.s8[ip], sip1 = S.s8[ip + 1];
-                    bool fm1_pending = (l1 == 0);
-                    while (true) {
-                        if (fm1_pending && (bits & 1u) == 0u) {
-                            fm1_pending = false;
-                            if (fm1o_src > CF_HALF) {
-                                float v = fm1o_src + fcab;
-                                v = v + jAx;
-                                v = v + mp;
-                                v = v + bpab;
-                                v = v - Z;
-                                post = post + cf_fexp(v, E);
-                            }
-                        }
-                        if (bits == 0u) break;
-                        const int k = __ffs(bits) - 1;
-                        bits &= bits - 1u;
-                        const int jp = qbase + k - 1;
-                        const int l2 = k;
-                        const int sl = (s + l1 + l2 + 2) % CF_RING;
-                        const float fco = RING ? ring[sl * RS + ip] : fco_t[cf_dbase(L, s + l1 + l2 + 2) + ip];
-                        const float outside = fco - Z;
-                        if (outside > CF_HALF) {
-                            const int sjp = S.s8[jp];
-                            float e;
-                            if (l1 + l2 == 0) {
-                                e = outside + bpab;
-                                e = e + T[CF_O_STACK + cf_i4(sam, sb2, sa, sb1)];
-                                e = e + fcab;
-                            } else {
-                                e = outside + S.JB[cf_i4(sip, S.s8[jp + 1], sip1, sjp)];
-                                e = e + S.cs[l1 * 31 + l2];
-                                e = e + fcab;
-                                e = e + bpab;
-                                e = e + jB2;
-                                const float sn = (l1 == 0 && l2 == 1) ? T[CF_O_B01 + sjp]
-                                               : (l1 == 1 && l2 == 0) ? T[CF_O_B01 + sip1]
-                                               : (l1 == 1 && l2 == 1) ? T[CF_O_I11 + cf_i2(sip1, sjp)] : 0.0f;
-                                e = e + sn;
-                            }
-                            post = post + cf_fexp(e, E);
-                        }
-                        if (fm1_pending && k == 0) {
-                            fm1_pending = false;
-                            if (fm1o_src > CF_HALF) {
-                                float v = fm1o_src + fcab;
-                                v = v + jAx;
-                                v = v + mp;
-                                v = v + bpab;
-                                v = v - Z;
-                                post = post + cf_fexp(v, E);
-                            }
-                        }
-                    }
-                }
-                const float outx = f5o[b + 1] - Z;
-                if (outx > CF_HALF && f5a > CF_HALF) {
-                    float v = outx + f5a;
-                    v = v + fcab;
-                    v = v + ep;
-                    v = v + bpab;
-                    v = v + jAx;
-                    post = post + cf_fexp(v, E);
-                }
-                post = post < 0.0f ? 0.0f : (post > 1.0f ? 1.0f : post);
-            }
-            post_t[db + a] = post;
         }
         cf_team_sync<PACK>();
         // ---- phase 3: FM2o and FM1o for every multiloop cell
@@ -813,28 +756,33 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             // (ii) FM2o(a, b+v) + FM[b][b+v] for v = L-1-b .. 2 descending (lanes step v together)
             const int vmax = L - 1 - (i0 + s);
             int v = vmax;
+            int ox = (int)cf_dbase(L, s + v), oy = (int)cf_dbase(L, v);  // diagonals s + v (FM2o) and v (FM)
             for (; v - 7 >= 2; v -= 8) {
                 float x[8], y[8];
+                bool g[8];
 #pragma unroll
                 for (int w = 0; w < 8; ++w) {
                     const int vv = v - w;
-                    x[w] = CF_NEG;
-                    y[w] = CF_NEG;
-                    if (dom && vv <= L - 1 - b) {
-                        x[w] = fm2o_t[cf_dbase(L, s + vv) + a];
-                        y[w] = fm_t[cf_dbase(L, vv) + b];
-                    }
+                    g[w] = dom & (vv <= L - 1 - b);
+                    x[w] = fm2o_t[ox + (g[w] ? a : 0)];
+                    y[w] = fm_t[oy + (g[w] ? b : 0)];
+                    ox -= L + 2 - (s + vv);
+                    oy -= L + 2 - vv;
                 }
 #pragma unroll
-                for (int w = 0; w < 8; ++w)
-                    f1 = cf_lse(f1, (x[w] > CF_HALF && y[w] > CF_HALF) ? x[w] + y[w] : CF_NEG, C);
+                for (int w = 0; w < 8; ++w) {
+                    const bool ok = g[w] & (x[w] > CF_HALF) & (y[w] > CF_HALF);
+                    f1 = cf_lse(f1, ok ? x[w] + y[w] : CF_NEG, C);
+                }
             }
             for (; v >= 2; --v) {
-                if (dom && v <= L - 1 - b) {
-                    const float x = fm2o_t[cf_dbase(L, s + v) + a];
-                    const float y = fm_t[cf_dbase(L, v) + b];
-                    if (x > CF_HALF && y > CF_HALF) f1 = cf_lse(f1, x + y, C);
-                }
+                const bool g = dom & (v <= L - 1 - b);
+                const float x = fm2o_t[ox + (g ? a : 0)];
+                const float y = fm_t[oy + (g ? b : 0)];
+                ox -= L + 2 - (s + v);
+                oy -= L + 2 - v;
+                const bool ok = g & (x > CF_HALF) & (y > CF_HALF);
+                f1 = cf_lse(f1, ok ? x + y : CF_NEG, C);
             }
             if (dom) {
                 const float fmo = fmod[a];
@@ -846,6 +794,157 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             }
         }
         cf_team_sync<PACK>();
+    }
+}
+
+// ================================================================ posterior
+// Base-pair posteriors (cpu.py:765-819) from the finished outside tables.
+// POST(a, b+1) sums exp(outside + inside - logZ) over the outer pairs that
+// close (a, b+1) as a single-branch loop, the multiloop (FM1) source and the
+// exterior source, in the reference's order.  Unlike FCo, the cells are
+// independent once the outside sweep is done, so each wave walks all cells of
+// its sequence with no synchronisation: the pairable ones are compacted into
+// a queue by ballot and evaluated a full wave (64 cells) at a time.  With
+// split > 1 the waves of a workgroup share one sequence and take every
+// split-th diagonal (lower latency for a lone long fold).
+// POSTF is indexed by FC cell: POSTF[tri(a, b)] = P(a pairs b+1).
+__global__ void __launch_bounds__(256) cf_post_kernel(
+    const CfSeq* __restrict__ seqs, const int* __restrict__ order, int nseq, const uint8_t* __restrict__ codes,
+    const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
+    const float* __restrict__ F5, const float* __restrict__ F5o, const float* __restrict__ FCo,
+    const float* __restrict__ FM1o, float* __restrict__ POSTF, int Lmax, int split) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int G = (int)(blockDim.x / WAVE);
+    const CfTeam tm = cf_team<true>(G);
+    const CfLds plan = cf_lds_plan(Lmax, false, 4, G);
+    const CfShared S = cf_carve(smem, plan, tm.team);
+    cf_stage_params(S, P);
+    __syncthreads();
+    const int li = (split > 1) ? (int)blockIdx.x : tm.li;
+    if (li >= nseq) return;
+    const CfSeq sd = seqs[order[li]];
+    const int L = sd.L;
+    const int lane = tm.tid;
+    cf_stage_seq<true>(S, codes, masks, sd, lane, WAVE);
+
+    const float mp = P->t[CF_O_MP], ep = P->t[CF_O_EP];
+    const float* __restrict__ T = P->t;
+    const float* __restrict__ fc_t = FC + sd.toff;
+    const float* __restrict__ fco_t = FCo + sd.toff;
+    const float* __restrict__ fm1o_t = FM1o + sd.toff;
+    const float* __restrict__ f5 = F5 + sd.coff;
+    const float* __restrict__ f5o = F5o + sd.coff;
+    float* __restrict__ post_t = POSTF + sd.toff;
+    const float4* __restrict__ E = S.ex;
+    int* __restrict__ queue = (int*)S.dbuf;
+    const float Z = f5[L];
+
+    auto cell = [&](int a, int s) {
+        const int b = a + s;
+        const float fcab = fc_t[S.dtab[s] + a];
+        float post = 0.0f;
+        if (fcab > CF_HALF) {
+            const int sa = S.s8[a], sb1 = S.s8[b + 1], sam = S.s8[a - 1], sb2 = S.s8[b + 2];
+            const float bpab = S.bp[cf_i2(sa, sb1)];
+            float jAx = T[CF_O_HC + cf_i2(sb1, sa)];
+            if (b + 1 < L) jAx = jAx + T[CF_O_DL + cf_i3(sb1, sa, sb2)];
+            if (a - 1 > 0) jAx = jAx + T[CF_O_DR + cf_i3(sb1, sa, sam)];
+            const float jB2 = S.JB[cf_i4(sb1, sa, sb2, sam)];
+            const float stk = T[CF_O_STACK + cf_i4(sam, sb2, sa, sb1)];
+            const float sn01 = T[CF_O_B01 + sb2], sn10 = T[CF_O_B01 + sam], sn11 = T[CF_O_I11 + cf_i2(sam, sb2)];
+            const bool fm1dom = (a - 1 > 0) && (a + 1 <= b + 1) && (b + 1 < L);
+            const float fm1o_src = fm1dom ? fm1o_t[S.dtab[s + 2] + a - 1] : CF_NEG;
+            const int lmax = min(CF_CMAX, a - 2);
+            const int qbase = b + 2;
+            // rows l1 = 0 .. lmax, partners ascending (the reverse of the FCo
+            // order); in row 0 the FM1o source follows the stacking term k = 0
+            for (int l1 = 0; l1 <= lmax; ++l1) {
+                const int ip = a - 1 - l1;
+                uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
+                const int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
+                auto term = [&](int l2) -> float {
+                    const int d = s + l1 + l2 + 2;
+                    const float outside = fco_t[S.dtab[d] + ip] - Z;
+                    float e;
+                    if (l1 + l2 == 0) {
+                        e = outside + bpab;
+                        e = e + stk;
+                        e = e + fcab;
+                    } else {
+                        e = outside + S.JB[rowJ + S.qo[qbase + l2 - 1]];
+                        e = e + S.cs[l1 * 31 + l2];
+                        e = e + fcab;
+                        e = e + bpab;
+                        e = e + jB2;
+                        const float sn = (l1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
+                                       : (l1 == 1) ? (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f)) : 0.0f;
+                        e = e + sn;
+                    }
+                    return (outside > CF_HALF) ? cf_fexp(e, E) : 0.0f;
+                };
+                if (l1 == 0) {
+                    if (bits & 1u) post = post + term(0);
+                    bits &= ~1u;
+                    if (fm1o_src > CF_HALF) {
+                        float v = fm1o_src + fcab;
+                        v = v + jAx;
+                        v = v + mp;
+                        v = v + bpab;
+                        v = v - Z;
+                        post = post + cf_fexp(v, E);
+                    }
+                }
+                while (bits != 0u) {
+                    const int kk = __ffs(bits) - 1;
+                    bits &= bits - 1u;
+                    post = post + term(kk);
+                }
+            }
+            const float outx = f5o[b + 1] - Z;
+            const float f5a = f5[a - 1];
+            if (outx > CF_HALF && f5a > CF_HALF) {
+                float v = outx + f5a;
+                v = v + fcab;
+                v = v + ep;
+                v = v + bpab;
+                v = v + jAx;
+                post = post + cf_fexp(v, E);
+            }
+            post = post < 0.0f ? 0.0f : (post > 1.0f ? 1.0f : post);
+        }
+        post_t[S.dtab[s] + a] = post;
+    };
+
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int qn = 0;  // queued cells (wave-uniform)
+    for (int s = (split > 1) ? tm.team : 0; s <= L; s += split) {
+        const int ncell = L + 1 - s;
+        for (int i0 = 0; i0 < ncell; i0 += WAVE) {
+            const int a = i0 + lane;
+            const int b = a + s;
+            const bool live = a < ncell;
+            const bool pr = live && (a > 0) && (b < L) && cf_allow(S, a, b + 1);
+            if (live && !pr) post_t[S.dtab[s] + a] = 0.0f;
+            const uint64_t m = __ballot(pr);
+            if (pr) queue[qn + __popcll(m & below)] = (a << 16) | s;
+            qn += __popcll(m);
+            if (qn >= WAVE) {
+                cf_team_sync<true>();
+                const int e = queue[lane];
+                cell(e >> 16, e & 0xffff);
+                cf_team_sync<true>();
+                const int rest = qn - WAVE;
+                const int moved = (lane < rest) ? queue[WAVE + lane] : 0;
+                cf_team_sync<true>();
+                if (lane < rest) queue[lane] = moved;
+                qn = rest;
+            }
+        }
+    }
+    cf_team_sync<true>();
+    if (lane < qn) {
+        const int e = queue[lane];
+        cell(e >> 16, e & 0xffff);
     }
 }
 
@@ -1741,15 +1840,15 @@ template __global__ void cf_inside_kernel<false, true>(const CfSeq*, const int*,
 template __global__ void cf_outside_kernel<true, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                         const CfParams<float>*, const float*, const float*, const float*,
                                                         const float*, const float*, float*, float*, float*, float*,
-                                                        float*, int);
+                                                        int);
 template __global__ void cf_outside_kernel<false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                          const CfParams<float>*, const float*, const float*,
                                                          const float*, const float*, const float*, float*, float*,
-                                                         float*, float*, float*, int);
+                                                         float*, float*, int);
 template __global__ void cf_outside_kernel<false, true>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                         const CfParams<float>*, const float*, const float*, const float*,
                                                         const float*, const float*, float*, float*, float*, float*,
-                                                        float*, int);
+                                                        int);
 template __global__ void cf_viterbi_kernel<true>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
                                                  const CfParams<double>*, double*, double*, double*, double*, uint8_t*,
                                                  const int64_t*, int);
