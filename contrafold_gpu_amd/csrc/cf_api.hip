@@ -96,6 +96,14 @@ int vit_lds(int Lmax) {
 // tables.  Wider buckets give the whole workgroup to one sequence.
 constexpr int kPack = 4;
 
+// Workgroup width of the kernels that have no packed layout (MEA, Viterbi,
+// gradient): a packed bucket (width 64) gives them a whole workgroup per
+// sequence, sized to the bucket's longest sequence.
+int solo_threads(const cf_batch::Bucket& bk) {
+    if (bk.threads > 64) return bk.threads;
+    return bk.Lmax <= 96 ? 128 : 256;
+}
+
 int launch_inside(cf_batch* b) {
     cf_ctx* c = b->ctx;
     for (const auto& bk : b->buckets) {
@@ -124,9 +132,15 @@ int launch_inside(cf_batch* b) {
 
 int launch_exterior(cf_batch* b, int mode) {
     cf_ctx* c = b->ctx;
-    const int grid = (b->n + 63) / 64;
-    hipLaunchKernelGGL(cf_exterior_kernel, dim3(grid), dim3(128), 0, b->stream, b->d_seqs, b->d_order, b->n,
-                       b->d_codes, b->d_masks, c->dP32, b->FCX, b->F5, b->F5o, b->logz, mode);
+    // one wave per (sequence, chain); up to four waves per workgroup as LDS allows
+    const int per = cf_ext_wave_bytes(b->Lmax);
+    const int avail = c->lds_max - 2048;  // static tables
+    if (per > avail) return fail(c, CF_EINVAL, "sequence too long for the exterior kernel");
+    const int nw = std::max(1, std::min(4, avail / per));
+    const int tasks = b->n * (mode == 3 ? 2 : 1);
+    hipLaunchKernelGGL(cf_exterior_kernel, dim3((tasks + nw - 1) / nw), dim3(64 * nw), nw * per, b->stream, b->d_seqs,
+                       b->d_order, b->n, b->d_codes, b->d_masks, c->dP32, b->FCX, b->F5, b->F5o, b->logz, mode,
+                       b->Lmax);
     CF_HIP(c, hipGetLastError(), "exterior launch");
     return CF_OK;
 }
@@ -169,9 +183,10 @@ int launch_post(cf_batch* b) {
     for (const auto& bk : b->buckets) {
         const int nl = bk.end - bk.begin;
         if (nl <= 0) continue;
-        // packed buckets: kPack sequences per workgroup; wide buckets: the
-        // workgroup's kPack waves split one sequence's diagonals
-        const bool pack = bk.threads <= 64;
+        // The kPack waves of a workgroup split one sequence's diagonals: the
+        // kernel waits on global FCo reads, and four times the waves of the
+        // packed form hide that better (12 vs 22 ms for 3000 x 200 nt).
+        const bool pack = false;
         const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, kPack);
         if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the posterior kernel's LDS plan");
         hipLaunchKernelGGL(cf_post_kernel, dim3(pack ? (nl + kPack - 1) / kPack : nl), dim3(64 * kPack), plan.total,
@@ -189,7 +204,7 @@ int launch_mea(cf_batch* b) {
         if (nl <= 0) continue;
         const int lds = mea_lds(bk.Lmax);
         if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the MEA kernel");
-        hipLaunchKernelGGL(cf_mea_kernel, dim3(nl * b->n_gamma), dim3(bk.threads), lds, b->stream, b->d_seqs,
+        hipLaunchKernelGGL(cf_mea_kernel, dim3(nl * b->n_gamma), dim3(solo_threads(bk)), lds, b->stream, b->d_seqs,
                            b->d_order + bk.begin, nl, b->d_codes, b->d_masks, b->POST, b->d_gin, b->SC, b->TB,
                            b->total_T, b->mea_out, b->d_charoff, b->total_chars, bk.Lmax);
         CF_HIP(c, hipGetLastError(), "mea launch");
@@ -205,11 +220,11 @@ int launch_viterbi(cf_batch* b, bool lse) {
         const int lds = vit_lds(bk.Lmax);
         if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the Viterbi kernel");
         if (lse)
-            hipLaunchKernelGGL(cf_viterbi_kernel<true>, dim3(nl), dim3(bk.threads), lds, b->stream, b->d_seqs,
+            hipLaunchKernelGGL(cf_viterbi_kernel<true>, dim3(nl), dim3(solo_threads(bk)), lds, b->stream, b->d_seqs,
                                b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP64, b->VFC, b->VFM, b->VFM1,
                                b->logz64, b->mfe_out, b->d_charoff, bk.Lmax);
         else
-            hipLaunchKernelGGL(cf_viterbi_kernel<false>, dim3(nl), dim3(bk.threads), lds, b->stream, b->d_seqs,
+            hipLaunchKernelGGL(cf_viterbi_kernel<false>, dim3(nl), dim3(solo_threads(bk)), lds, b->stream, b->d_seqs,
                                b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP64, b->VFC, b->VFM, b->VFM1,
                                b->vscore, b->mfe_out, b->d_charoff, bk.Lmax);
         CF_HIP(c, hipGetLastError(), "viterbi launch");
@@ -232,7 +247,7 @@ int launch_grad(cf_batch* b) {
     for (const auto& bk : b->buckets) {
         const int nl = bk.end - bk.begin;
         if (nl <= 0) continue;
-        hipLaunchKernelGGL(cf_grad_kernel, dim3(nl), dim3(bk.threads), 0, b->stream, b->d_seqs,
+        hipLaunchKernelGGL(cf_grad_kernel, dim3(nl), dim3(solo_threads(bk)), 0, b->stream, b->d_seqs,
                            b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
                            b->F5, b->FCo, b->FMo, b->FM1o, b->F5o, (b->d_gin + kMaxGamma), b->grad);
         CF_HIP(c, hipGetLastError(), "grad launch");
@@ -387,7 +402,7 @@ size_t batch_staging_bytes(int n, int64_t C) {
 
 int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
                  const uint64_t* keys, hipStream_t stream, void** staging, size_t* staging_bytes, bool sync,
-                 cf_batch** out) {
+                 cf_batch** out, int width) {
     if (!c || !out) return fail(c, CF_EINVAL, "null argument");
     *out = nullptr;
     if (n <= 0) return fail(c, CF_EINVAL, "empty batch");
@@ -422,7 +437,7 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
             const int lo = k ? c->edges[k - 1] : -1;
             int end = pos;
             while (end < n && b->L[b->order[end]] > lo) ++end;
-            if (end > pos) b->buckets.push_back({pos, end, c->threads[k], b->L[b->order[pos]]});
+            if (end > pos) b->buckets.push_back({pos, end, width > 0 ? width : c->threads[k], b->L[b->order[pos]]});
             pos = end;
         }
     }
@@ -638,6 +653,7 @@ int cf_create(int device, cf_ctx** out) {
     const void* dyn[] = {(const void*)cf_inside_kernel<false, false>, (const void*)cf_inside_kernel<false, true>,
                          (const void*)cf_outside_kernel<true, false>, (const void*)cf_outside_kernel<false, false>,
                          (const void*)cf_outside_kernel<false, true>, (const void*)cf_post_kernel,
+                         (const void*)cf_exterior_kernel,
                          (const void*)cf_mea_kernel, (const void*)cf_viterbi_kernel<false>,
                          (const void*)cf_viterbi_kernel<true>};
     for (const void* f : dyn) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);

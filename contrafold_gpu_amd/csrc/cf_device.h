@@ -64,6 +64,11 @@ __host__ __device__ inline int64_t cf_dbase(int L, int d) {
     return (int64_t)d * (L + 1) - (int64_t)d * (d - 1) / 2;
 }
 __host__ __device__ inline int64_t cf_tri(int L, int i, int j) { return cf_dbase(L, j - i) + i; }
+// 32-bit form for per-sequence offsets (exact for L <= 32000: at most ~5.1e8)
+__host__ __device__ inline int cf_tri32(int L, int i, int j) {
+    const int d = j - i;
+    return d * (L + 1) - ((d * (d - 1)) >> 1) + i;
+}
 __host__ __device__ inline int64_t cf_tri_size(int L) { return (int64_t)(L + 1) * (L + 2) / 2; }
 // Exterior (column-major) copy of FC used by the F5 / F5o chains: entry
 // (k, j), 0 <= k < j <= L, holds FC[k+1][j-1].

@@ -16,11 +16,16 @@ static const char* kKernelNames[] = {"inside", "exterior", "outside", "post", "m
 constexpr int kNumKernels = 8;
 enum { K_INSIDE = 0, K_EXTERIOR, K_OUTSIDE, K_POST, K_MEA, K_VITERBI, K_SAMPLE, K_GRAD };
 
-// Caching device allocator: blocks are rounded to 2 MiB and kept on a free
-// list after release, so steady-state batches do not call hipMalloc.
+// Caching device allocator: block sizes are rounded up to a power of two
+// (at least 2 MiB) and kept on a free list after release, so steady-state
+// batches of varying size reuse blocks instead of calling hipMalloc.
 struct DevPool {
     std::multimap<size_t, void*> free_;
-    static size_t round(size_t n) { return (n + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1); }
+    static size_t round(size_t n) {
+        size_t r = size_t(2) << 20;
+        while (r < n) r <<= 1;
+        return r;
+    }
     hipError_t get(void** p, size_t n) {
         n = round(n ? n : 1);
         auto it = free_.find(n);
@@ -137,9 +142,10 @@ int fail(cf_ctx* c, int code, const std::string& msg);
 // Pack and upload a batch on `stream` from the pinned buffer `staging`
 // (which must stay untouched until the stream has passed the upload).  With
 // sync = true the upload is waited for and `staging` may be reused at once.
+// width > 0 overrides the tuned team width of every bucket.
 int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
                  const uint64_t* keys, hipStream_t stream, void** staging, size_t* staging_bytes, bool sync,
-                 cf_batch** out);
+                 cf_batch** out, int width = 0);
 // Bytes of pinned staging batch_create needs for these sizes.
 size_t batch_staging_bytes(int n, int64_t total_residues);
 // Prepare buffers and per-run inputs, then enqueue the stages of `flags` on

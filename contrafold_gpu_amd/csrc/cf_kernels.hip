@@ -402,19 +402,49 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
 }
 
 // ================================================================== exterior
-// The exterior chains, one lane per sequence (cpu.py:666-677 and 684-699):
-// threads 0..63 of a block fold F5 for 64 sequences, threads 64..127 fold the
-// outside F5o for the same sequences, in the reference's order.  FC is read
-// from the column-major copy FCX, which each lane streams sequentially.
-__global__ void __launch_bounds__(128) cf_exterior_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
+// The exterior chains (cpu.py:666-677 and 684-699), one wave per (sequence,
+// chain).  The sequence, its mask and the chain's values live in LDS.
+//   F5:  F5[j] folds F5[j-1] + eu and then the terms k = 0 .. j-1 in order.
+//        The lanes evaluate 64 terms at once (coalesced reads of the FCX
+//        column); the fold itself is serial and runs on wave-uniform values:
+//        only the terms that exist are visited (folding NEG is the identity,
+//        so skipping them leaves every bit of the result unchanged).
+//   F5o: step j (descending) adds one source to each F5o[k], k < j; those
+//        updates are independent, so lanes take one k each and the chain is
+//        L steps long instead of L^2/2.
+// Log-sum-exp on uniform values: the segment index is made scalar and the
+// cubic's coefficients come from the scalar cache.
+__device__ __forceinline__ float cf_lse_u(float a, float b) {
+    const float hi = fmaxf(a, b);
+    const float lo = fminf(a, b);
+    const float d = hi - lo;
+    const bool b2 = d >= 3.3792499610f;
+    const bool b1 = d >= (b2 ? 5.7890710412f : 1.6320158198f);
+    const float t0 = b2 ? (b1 ? 7.8162726752f : 4.4261691294f) : (b1 ? 2.4912588184f : 0.6615367791f);
+    const bool b0 = d >= t0;
+    const int seg = __builtin_amdgcn_readfirstlane((b2 ? 4 : 0) | (b1 ? 2 : 0) | (b0 ? 1 : 0));
+    float r = cf_lse_coef[seg][0] * d;
+    r = r + cf_lse_coef[seg][1];
+    r = r * d;
+    r = r + cf_lse_coef[seg][2];
+    r = r * d;
+    r = r + cf_lse_coef[seg][3];
+    const float res = lo + r;
+    return (lo < CF_HALF || d >= 11.8624794162f) ? hi : res;
+}
+
+__host__ __device__ inline int cf_ext_wave_bytes(int Lmax) { return cf_align((Lmax + 2) * 4 + 2 * (Lmax + 3), 16); }
+
+__global__ void __launch_bounds__(256) cf_exterior_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
                                                           int n, const uint8_t* __restrict__ codes,
                                                           const uint8_t* __restrict__ masks,
                                                           const CfParams<float>* __restrict__ P,
                                                           const float* __restrict__ FCX, float* __restrict__ F5,
                                                           float* __restrict__ F5o, float* __restrict__ logz,
-                                                          int mode) {
+                                                          int mode, int Lmax) {
     __shared__ float4 C[8];
     __shared__ float tabs[25 + 25 + 125 + 125];
+    extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     if (tid < 8) C[tid] = make_float4(cf_lse_coef[tid][0], cf_lse_coef[tid][1], cf_lse_coef[tid][2], cf_lse_coef[tid][3]);
     for (int t = tid; t < 300; t += blockDim.x) {
@@ -430,116 +460,99 @@ __global__ void __launch_bounds__(128) cf_exterior_kernel(const CfSeq* __restric
     const float* hc = tabs + 25;
     const float* dl = tabs + 50;
     const float* dr = tabs + 175;
-    const bool outside = tid >= 64;  // mode bit 0: F5 chain, bit 1: F5o chain
-    if (outside ? !(mode & 2) : !(mode & 1)) return;
-    const int li = blockIdx.x * 64 + (tid & 63);
-    if (li >= n) return;
+    const int nw = (int)(blockDim.x / WAVE);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int lane = tid & (WAVE - 1);
+    const int nchain = (mode == 3) ? 2 : 1;
+    const int task = blockIdx.x * nw + w;
+    if (task >= n * nchain) return;
+    const int li = task / nchain;
+    const bool outside = (mode == 3) ? (task % nchain) == 1 : (mode == 2);
     const int b = order[li];
     const CfSeq sd = seqs[b];
     const int L = sd.L;
-    const uint8_t* __restrict__ s = codes + sd.coff;
-    const uint8_t* __restrict__ fo = masks ? masks + sd.coff : nullptr;
+    unsigned char* base = smem + w * cf_ext_wave_bytes(Lmax);
+    float* __restrict__ f = (float*)base;
+    int8_t* __restrict__ s8 = (int8_t*)(base + (Lmax + 2) * 4);
+    int8_t* __restrict__ fo8 = s8 + (Lmax + 3);
+    for (int q = lane; q < L + 2; q += WAVE) {
+        s8[q] = (int8_t)codes[sd.coff + q];
+        fo8[q] = (masks != nullptr && masks[sd.coff + q] != 0) ? 1 : 0;
+    }
     const float* __restrict__ fcx = FCX + sd.xoff;
     const float eu = P->t[CF_O_EU], ep = P->t[CF_O_EP];
-    auto forced = [&](int q) -> bool { return fo != nullptr && fo[q] != 0; };
-    // k-loops run 8 terms per step: the loads of a step are issued together
-    // and the terms are then folded (F5) or scattered (F5o) in k order.
     if (!outside) {
-        float* __restrict__ f5 = F5 + sd.coff;
-        f5[0] = 0.0f;
+        if (lane == 0) f[0] = 0.0f;
+        cf_team_sync<true>();
         for (int j = 1; j <= L; ++j) {
-            float acc = f5[j - 1] + eu;
-            const int sj = s[j];
-            const int sj1 = s[j + 1];
-            if (!forced(j) && sj != 4) {
+            float acc = f[j - 1] + eu;
+            const int sj = s8[j];
+            const int sj1 = s8[j + 1];
+            if (!fo8[j] && sj != 4) {
                 const float* __restrict__ col = fcx + cf_xbase(j);
-                for (int k0 = 0; k0 < j; k0 += 8) {
-                    float fcv[8], f5v[8];
-                    int sk1v[8], skv[8];
-                    bool ok[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int k = k0 + u;
-                        ok[u] = false;
-                        fcv[u] = CF_NEG;
-                        f5v[u] = CF_NEG;
-                        sk1v[u] = 4;
-                        skv[u] = 4;
-                        if (k < j) {
-                            sk1v[u] = s[k + 1];
-                            skv[u] = s[k];
-                            ok[u] = cf_canon(sk1v[u], sj) && !forced(k + 1);
-                            fcv[u] = col[k];
-                            f5v[u] = f5[k];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int k = k0 + u;
-                        float v = CF_NEG;
-                        if (ok[u] && fcv[u] > CF_HALF && f5v[u] > CF_HALF) {
-                            float jA = hc[cf_i2(sj, sk1v[u])];
-                            if (j < L) jA = jA + dl[cf_i3(sj, sk1v[u], sj1)];
-                            if (k > 0) jA = jA + dr[cf_i3(sj, sk1v[u], skv[u])];
-                            v = f5v[u] + fcv[u];
+                for (int k0 = 0; k0 < j; k0 += WAVE) {
+                    const int k = k0 + lane;
+                    float v = CF_NEG;
+                    if (k < j) {
+                        const int sk1 = s8[k + 1], sk = s8[k];
+                        const float fcv = col[k];
+                        const float f5k = f[k];
+                        if (cf_canon(sk1, sj) && !fo8[k + 1] && fcv > CF_HALF && f5k > CF_HALF) {
+                            float jA = hc[cf_i2(sj, sk1)];
+                            if (j < L) jA = jA + dl[cf_i3(sj, sk1, sj1)];
+                            if (k > 0) jA = jA + dr[cf_i3(sj, sk1, sk)];
+                            v = f5k + fcv;
                             v = v + ep;
-                            v = v + bp[cf_i2(sk1v[u], sj)];
+                            v = v + bp[cf_i2(sk1, sj)];
                             v = v + jA;
                         }
-                        acc = cf_lse(acc, v, C);
+                    }
+                    uint64_t m = __ballot(v > CF_HALF);
+                    while (m) {
+                        const int t = __ffsll((unsigned long long)m) - 1;
+                        m &= m - 1;
+                        acc = cf_lse_u(acc, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t)));
                     }
                 }
             }
-            f5[j] = acc;
+            if (lane == 0) f[j] = acc;
+            cf_team_sync<true>();
         }
-        f5[L + 1] = CF_NEG;
-        logz[b] = f5[L];
+        if (lane == 0) f[L + 1] = CF_NEG;
+        cf_team_sync<true>();
+        float* __restrict__ f5 = F5 + sd.coff;
+        for (int q = lane; q < L + 2; q += WAVE) f5[q] = f[q];
+        if (lane == 0) logz[b] = f[L];
     } else {
-        float* __restrict__ f5o = F5o + sd.coff;
-        for (int k = 0; k <= L + 1; ++k) f5o[k] = CF_NEG;
-        f5o[L] = 0.0f;
+        for (int q = lane; q < L + 2; q += WAVE) f[q] = (q == L) ? 0.0f : CF_NEG;
+        cf_team_sync<true>();
         for (int j = L; j >= 1; --j) {
-            const float f5oj = f5o[j];
-            f5o[j - 1] = cf_lse(f5o[j - 1], f5oj + eu, C);
-            const int sj = s[j];
-            const int sj1 = s[j + 1];
-            if (forced(j) || sj == 4) continue;
-            const float* __restrict__ col = fcx + cf_xbase(j);
-            for (int k0 = 0; k0 < j; k0 += 8) {
-                float fcv[8], f5ov[8];
-                int sk1v[8], skv[8];
-                bool ok[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int k = k0 + u;
-                    ok[u] = false;
-                    fcv[u] = CF_NEG;
-                    f5ov[u] = CF_NEG;
-                    sk1v[u] = 4;
-                    skv[u] = 4;
-                    if (k < j) {
-                        sk1v[u] = s[k + 1];
-                        skv[u] = s[k];
-                        ok[u] = cf_canon(sk1v[u], sj) && !forced(k + 1);
-                        fcv[u] = col[k];
-                        f5ov[u] = f5o[k];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int k = k0 + u;
-                    if (ok[u] && fcv[u] > CF_HALF) {
-                        float jA = hc[cf_i2(sj, sk1v[u])];
-                        if (j < L) jA = jA + dl[cf_i3(sj, sk1v[u], sj1)];
-                        if (k > 0) jA = jA + dr[cf_i3(sj, sk1v[u], skv[u])];
+            const float f5oj = f[j];
+            const float prev = cf_lse_u(f[j - 1], f5oj + eu);
+            if (lane == 0) f[j - 1] = prev;
+            cf_team_sync<true>();
+            const int sj = s8[j];
+            const int sj1 = s8[j + 1];
+            if (!fo8[j] && sj != 4) {
+                const float* __restrict__ col = fcx + cf_xbase(j);
+                for (int k = lane; k < j; k += WAVE) {
+                    const int sk1 = s8[k + 1], sk = s8[k];
+                    const float fcv = col[k];
+                    if (cf_canon(sk1, sj) && !fo8[k + 1] && fcv > CF_HALF) {
+                        float jA = hc[cf_i2(sj, sk1)];
+                        if (j < L) jA = jA + dl[cf_i3(sj, sk1, sj1)];
+                        if (k > 0) jA = jA + dr[cf_i3(sj, sk1, sk)];
                         float temp = f5oj + ep;
-                        temp = temp + bp[cf_i2(sk1v[u], sj)];
+                        temp = temp + bp[cf_i2(sk1, sj)];
                         temp = temp + jA;
-                        f5o[k] = cf_lse(f5ov[u], temp + fcv[u], C);
+                        f[k] = cf_lse(f[k], temp + fcv, C);
                     }
                 }
+                cf_team_sync<true>();
             }
         }
+        float* __restrict__ f5o = F5o + sd.coff;
+        for (int q = lane; q < L + 2; q += WAVE) f5o[q] = f[q];
     }
 }
 
@@ -1469,7 +1482,8 @@ struct CfRng {
 };
 
 // One Boltzmann traceback per lane (the reference's state machine,
-// gpu.py:220-477, with candidate order unchanged).  Each state draws one
+// gpu.py:220-477, with candidate order unchanged; probabilities use the
+// hardware exp2, sampling is checked statistically, not bit for bit).  Each state draws one
 // uniform r and walks its candidates' probabilities exp(w - total), where
 // total is the state's own inside value; the multiloop options are grouped
 // through the stored split FM2 and resolved with a second draw.  When FP32
@@ -1480,11 +1494,17 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
     const CfParams<float>* __restrict__ P, const float* __restrict__ FC, const float* __restrict__ FM,
     const float* __restrict__ FM1, const float* __restrict__ FM2, const float* __restrict__ F5,
     uint32_t* __restrict__ stack, int stack_cap, uint8_t* __restrict__ out, const int64_t* __restrict__ out_off) {
+    // the whole parameter block in LDS: every candidate reads several entries
+    __shared__ CfParams<float> sp;
+    for (int t = threadIdx.x; t < CF_NTAB; t += blockDim.x) sp.t[t] = P->t[t];
+    for (int t = threadIdx.x; t < 625; t += blockDim.x) sp.JB[t] = P->JB[t];
+    __syncthreads();
     const int64_t total = (int64_t)nseq * nsamp;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t gsz = (int64_t)gridDim.x * blockDim.x;
     uint32_t* __restrict__ st = stack + gid * stack_cap;
-    const float* __restrict__ T = P->t;
+    const float* __restrict__ T = sp.t;
+    const float* __restrict__ JBt = sp.JB;
     const float mb = T[CF_O_MB], mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
     for (int64_t task = gid; task < total; task += gsz) {
         const int li = (int)(task / nsamp);
@@ -1524,20 +1544,20 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                 if (j == 0) continue;
                 const float tot = f5[j];
                 const float r = rng.next();
-                float c = expf(f5[j - 1] + eu - tot);
+                float c = __expf(f5[j - 1] + eu - tot);
                 int pick = -1;
                 if (!(r <= c)) {
                     const int sj = s[j];
                     for (int k = 0; k < j; ++k) {
                         if (!allow(k + 1, j)) continue;
-                        const float fc = fc_t[cf_tri(L, k + 1, j - 1)];
+                        const float fc = fc_t[cf_tri32(L, k + 1, j - 1)];
                         if (!(fc > CF_HALF) || !(f5[k] > CF_HALF)) continue;
                         const int sk1 = s[k + 1];
                         float jA = T[CF_O_HC + cf_i2(sj, sk1)];
                         if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s[j + 1])];
                         if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s[k])];
                         const float w = f5[k] + fc + ep + T[CF_O_BP + cf_i2(sk1, sj)] + jA;
-                        c += expf(w - tot);
+                        c += __expf(w - tot);
                         pick = k;
                         if (r <= c) break;
                     }
@@ -1551,12 +1571,12 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                     push(0, 0, pick);
                 }
             } else if (typ == 1) {  // FC(i, j): pair (i, j+1) already recorded
-                const float tot = fc_t[cf_tri(L, i, j)];
+                const float tot = fc_t[cf_tri32(L, i, j)];
                 const int si = s[i], sj1 = s[j + 1], si1 = s[i + 1], sj = s[j];
-                const float jB = P->JB[cf_i4(si, sj1, si1, sj)];
+                const float jB = JBt[cf_i4(si, sj1, si1, sj)];
                 const int d = j - i;
                 const float r = rng.next();
-                float c = expf(jB + T[CF_O_HP + (d <= 30 ? d : 30)] - tot);
+                float c = __expf(jB + T[CF_O_HP + (d <= 30 ? d : 30)] - tot);
                 if (r <= c) continue;  // hairpin
                 int pp = -1, pq = -1;
                 const int pmax = min(i + CF_CMAX, j);
@@ -1566,7 +1586,7 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                     const int qmin = max(p + 2, p - i + j - CF_CMAX);
                     for (int q = j; q >= qmin; --q) {
                         if (!allow(p + 1, q)) continue;
-                        const float fc = fc_t[cf_tri(L, p + 1, q - 1)];
+                        const float fc = fc_t[cf_tri32(L, p + 1, q - 1)];
                         if (!(fc > CF_HALF)) continue;
                         const int l2 = j - q;
                         float e;
@@ -1578,30 +1598,30 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                             else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
                             else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
                             e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s[p + 1], s[q])] + jB +
-                                P->JB[cf_i4(s[q], s[p + 1], s[q + 1], s[p])] + sn;
+                                JBt[cf_i4(s[q], s[p + 1], s[q + 1], s[p])] + sn;
                         }
-                        c += expf(e + fc - tot);
+                        c += __expf(e + fc - tot);
                         pp = p;
                         pq = q;
                         if (r <= c) { done = true; break; }
                     }
                 }
-                const float fm2 = fm2_t[cf_tri(L, i, j)];
+                const float fm2 = fm2_t[cf_tri32(L, i, j)];
                 if (!done && fm2 > CF_HALF) {
                     float jA = T[CF_O_HC + cf_i2(si, sj1)];
                     if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
                     if (j > 0) jA = jA + T[CF_O_DR + cf_i3(si, sj1, sj)];
                     const float wm = fm2 + jA + mp + mb;
-                    c += expf(wm - tot);
+                    c += __expf(wm - tot);
                     {  // chosen by r, or as the last candidate when round-off left r above the sum
                         // multiloop: choose the split k with weight FM1[i][k] + FM[k][j] over FM2[i][j]
                         const float r2 = rng.next();
                         float c2 = 0.0f;
                         int kk = -1;
                         for (int k = i + 1; k < j; ++k) {
-                            const float x = fm1_t[cf_tri(L, i, k)], y = fm_t[cf_tri(L, k, j)];
+                            const float x = fm1_t[cf_tri32(L, i, k)], y = fm_t[cf_tri32(L, k, j)];
                             if (!(x > CF_HALF && y > CF_HALF)) continue;
-                            c2 += expf(x + y - fm2);
+                            c2 += __expf(x + y - fm2);
                             kk = k;
                             if (r2 <= c2) break;
                         }
@@ -1618,25 +1638,25 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                     push(1, pp + 1, pq - 1);
                 }
             } else if (typ == 2) {  // FM(i, j): split, j unpaired, or one branch (in that order)
-                const float tot = fm_t[cf_tri(L, i, j)];
+                const float tot = fm_t[cf_tri32(L, i, j)];
                 const float r = rng.next();
-                const float fm2 = fm2_t[cf_tri(L, i, j)];
-                const float wun = fm_t[cf_tri(L, i, j - 1)];
-                const float w1 = fm1_t[cf_tri(L, i, j)];
+                const float fm2 = fm2_t[cf_tri32(L, i, j)];
+                const float wun = fm_t[cf_tri32(L, i, j - 1)];
+                const float w1 = fm1_t[cf_tri32(L, i, j)];
                 const bool h2 = fm2 > CF_HALF, hu = wun > CF_HALF, h1 = w1 > CF_HALF;
                 float c = 0.0f;
                 int opt = -1;
-                if (h2) { c += expf(fm2 - tot); if (r <= c || !(hu || h1)) opt = 0; }
-                if (opt < 0 && hu) { c += expf(wun + mu - tot); if (r <= c || !h1) opt = 1; }
+                if (h2) { c += __expf(fm2 - tot); if (r <= c || !(hu || h1)) opt = 0; }
+                if (opt < 0 && hu) { c += __expf(wun + mu - tot); if (r <= c || !h1) opt = 1; }
                 if (opt < 0 && h1) opt = 2;
                 if (opt == 0) {
                     const float r2 = rng.next();
                     float c2 = 0.0f;
                     int kk = -1;
                     for (int k = i + 1; k < j; ++k) {
-                        const float x = fm1_t[cf_tri(L, i, k)], y = fm_t[cf_tri(L, k, j)];
+                        const float x = fm1_t[cf_tri32(L, i, k)], y = fm_t[cf_tri32(L, k, j)];
                         if (!(x > CF_HALF && y > CF_HALF)) continue;
-                        c2 += expf(x + y - fm2);
+                        c2 += __expf(x + y - fm2);
                         kk = k;
                         if (r2 <= c2) break;
                     }
@@ -1647,19 +1667,19 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                     push(3, i, j);
                 }
             } else {  // FM1(i, j): pair (i+1, j) or i+1 unpaired
-                const float tot = fm1_t[cf_tri(L, i, j)];
+                const float tot = fm1_t[cf_tri32(L, i, j)];
                 const float r = rng.next();
                 float c = 0.0f;
                 bool paired = false;
                 if (allow(i + 1, j)) {
-                    const float fc = fc_t[cf_tri(L, i + 1, j - 1)];
+                    const float fc = fc_t[cf_tri32(L, i + 1, j - 1)];
                     if (fc > CF_HALF) {
                         const int si1 = s[i + 1], sj = s[j];
                         float jA = T[CF_O_HC + cf_i2(sj, si1)];
                         if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, s[j + 1])];
                         if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, s[i])];
-                        c += expf(fc + jA + mp + T[CF_O_BP + cf_i2(si1, sj)] - tot);
-                        const float wun = fm1_t[cf_tri(L, i + 1, j)];
+                        c += __expf(fc + jA + mp + T[CF_O_BP + cf_i2(si1, sj)] - tot);
+                        const float wun = fm1_t[cf_tri32(L, i + 1, j)];
                         paired = (r <= c) || !(wun > CF_HALF);
                     }
                 }
@@ -1667,7 +1687,7 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                     o[i] = '(';
                     o[j - 1] = ')';
                     push(1, i + 1, j - 1);
-                } else if (fm1_t[cf_tri(L, i + 1, j)] > CF_HALF) {
+                } else if (fm1_t[cf_tri32(L, i + 1, j)] > CF_HALF) {
                     push(3, i + 1, j);
                 }
             }

@@ -13,6 +13,7 @@
 // the context's pool.  A request's sampling stream is keyed by its own 64-bit
 // key (cf_server_submit's `seed`), so its result does not depend on the batch
 // it lands in.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -45,6 +46,8 @@ struct Slot {
     size_t stage_in_bytes = 0;
     void* stage_run = nullptr;
     size_t stage_run_bytes = 0;
+    void* out = nullptr;  // pinned: logZ then the kind's payload, copied on the slot's stream
+    size_t out_bytes = 0;
     cf_batch* batch = nullptr;
     std::vector<int64_t> reqs;
     unsigned kind = 0;
@@ -85,36 +88,36 @@ struct cf_server {
 
 namespace {
 
+// Results were copied to the slot's pinned buffer on the slot's stream before
+// its done event, so retiring a batch makes no blocking device call (a
+// synchronous copy would queue behind other batches' kernels on a shared
+// hardware queue).
 int retire(cf_server* s, Slot& sl) {
-    cf_ctx* c = s->ctx;
     cf_batch* b = sl.batch;
     const double t = s->now_us();
-    std::vector<float> lz(b->n);
-    int rc = cf_batch_logz(b, lz.data());
-    std::vector<char> mea, smp;
-    if (!rc && sl.kind == CF_MEA) {
-        mea.resize(b->total_chars);
-        rc = cf_batch_mea(b, 0, mea.data());
-    }
-    if (!rc && sl.kind == CF_SAMPLE) {
-        smp.resize((size_t)b->n_samples * b->total_chars);
-        rc = cf_batch_samples(b, smp.data());
-    }
+    const float* lz = (const float*)sl.out;
+    const char* pay = (const char*)sl.out + sizeof(float) * (size_t)b->n;
+    int64_t toff = 0;
     for (size_t k = 0; k < sl.reqs.size(); ++k) {
         Req& r = s->reqs[sl.reqs[k]];
-        r.state = rc ? 3 : 2;
+        r.state = 2;
         r.done_us = t;
-        r.logz = rc ? 0.0f : lz[k];
+        r.logz = lz[k];
         const int L = b->L[k];
-        if (!rc && sl.kind == CF_MEA) {
-            r.result.assign(mea.begin() + b->charoff[k], mea.begin() + b->charoff[k] + L);
-        } else if (!rc && sl.kind == CF_SAMPLE) {
+        if (sl.kind == CF_MEA) {
+            r.result.assign(pay + b->charoff[k], pay + b->charoff[k] + L);
+        } else if (sl.kind == CF_SAMPLE) {
             const int64_t off = b->charoff[k] * (int64_t)b->n_samples;
-            r.result.assign(smp.begin() + off, smp.begin() + off + (int64_t)b->n_samples * L);
-        } else if (!rc && sl.kind == CF_BPP) {
-            r.result.resize((size_t)L * L * sizeof(float));
-            if (L) rc = cf_batch_bpp(b, (int)k, (float*)r.result.data());
+            r.result.assign(pay + off, pay + off + (int64_t)b->n_samples * L);
+        } else if (sl.kind == CF_BPP) {
+            // POST is indexed by FC cell: entry (a, c-1) holds P(a pairs c)
+            const float* tri = (const float*)pay + toff;
+            r.result.assign((size_t)L * L * sizeof(float), 0);
+            float* m = (float*)r.result.data();
+            for (int a = 1; a <= L; ++a)
+                for (int c = a + 1; c <= L; ++c) m[(size_t)(a - 1) * L + (c - 1)] = tri[cf_tri(L, a, c - 1)];
         }
+        toff += cf_tri_size(L);
         s->completed.push_back(r.id);
     }
     cf_batch_destroy(b);
@@ -122,8 +125,7 @@ int retire(cf_server* s, Slot& sl) {
     sl.reqs.clear();
     s->in_use -= sl.bytes;
     sl.bytes = 0;
-    (void)c;
-    return rc;
+    return CF_OK;
 }
 
 int launch(cf_server* s, Slot& sl) {
@@ -183,9 +185,12 @@ int launch(cf_server* s, Slot& sl) {
         keys[k] = r.seed;  // the request's own stream key: independent of the batch it joins
         r.state = 1;
     }
+    // A small batch cannot fill the GPU with one wave per fold; giving each
+    // fold a whole 256-thread workgroup cuts its latency instead.
+    const int width = (n <= 4 * c->n_cu) ? 256 : 0;
     cf_batch* b = nullptr;
     int rc = cfi::batch_create(c, n, seqs.data(), offs.data(), any_mask ? masks.data() : nullptr, keys.data(),
-                               sl.stream, &sl.stage_in, &sl.stage_in_bytes, false, &b);
+                               sl.stream, &sl.stage_in, &sl.stage_in_bytes, false, &b, width);
     if (!rc) {
         const size_t need = sizeof(float) * (cfi::kMaxGamma + (size_t)n) + sizeof(int64_t) * (size_t)n + 4096;
         if (sl.stage_run_bytes < need) {
@@ -202,6 +207,30 @@ int launch(cf_server* s, Slot& sl) {
     if (!rc)
         rc = cfi::batch_enqueue(b, kind_flags(kind), &gamma, 1, nsamp > 0 ? nsamp : 1, 0ull, nullptr,
                                 sl.stage_run, sl.stage_run_bytes, false, nullptr);
+    if (!rc) {
+        // results -> pinned host memory, in stream order after the kernels
+        const void* src = nullptr;
+        size_t pay = 0;
+        if (kind == CF_MEA) src = b->mea_out, pay = (size_t)b->total_chars;
+        else if (kind == CF_SAMPLE) src = b->samp_out, pay = (size_t)b->n_samples * b->total_chars;
+        else if (kind == CF_BPP) src = b->POST, pay = sizeof(float) * (size_t)b->total_T;
+        const size_t need = sizeof(float) * (size_t)n + pay;
+        if (sl.out_bytes < need) {
+            if (sl.out) (void)hipHostFree(sl.out);  // the slot's stream is idle: its last batch was retired
+            sl.out = nullptr;
+            sl.out_bytes = 0;
+            const size_t cap = std::max(need, (size_t)1 << 20);
+            if (hipHostMalloc(&sl.out, cap) != hipSuccess) rc = cfi::fail(c, CF_ENOMEM, "pinned result buffer");
+            else sl.out_bytes = cap;
+        }
+        if (!rc && hipMemcpyAsync(sl.out, b->logz, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, sl.stream) !=
+                       hipSuccess)
+            rc = cfi::fail(c, CF_EDEVICE, "result copy");
+        if (!rc && pay &&
+            hipMemcpyAsync((char*)sl.out + sizeof(float) * (size_t)n, src, pay, hipMemcpyDeviceToHost, sl.stream) !=
+                hipSuccess)
+            rc = cfi::fail(c, CF_EDEVICE, "result copy");
+    }
     if (!rc && hipEventRecord(sl.done, sl.stream) != hipSuccess) rc = cfi::fail(c, CF_EDEVICE, "event record");
     if (rc) {
         if (b) cf_batch_destroy(b);
@@ -354,6 +383,7 @@ void cf_server_destroy(cf_server* s) {
         if (sl.batch) cf_batch_destroy(sl.batch);
         if (sl.stage_in) (void)hipHostFree(sl.stage_in);
         if (sl.stage_run) (void)hipHostFree(sl.stage_run);
+        if (sl.out) (void)hipHostFree(sl.out);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
