@@ -234,7 +234,7 @@ int launch_viterbi(cf_batch* b, bool lse) {
 
 int launch_sample(cf_batch* b, uint64_t seed) {
     cf_ctx* c = b->ctx;
-    hipLaunchKernelGGL(cf_sample_kernel, dim3(b->sgrid), dim3(64), 0, b->stream, b->d_seqs, b->d_order, b->n,
+    hipLaunchKernelGGL(cf_sample_kernel, dim3(b->sgrid), dim3(256), 0, b->stream, b->d_seqs, b->d_order, b->n,
                        b->n_samples, seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
                        b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff);
     CF_HIP(c, hipGetLastError(), "sample launch");
@@ -307,7 +307,9 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
     }
     if (flags & CF_SAMPLE) {
         const int64_t tasks = (int64_t)b->n * n_samples;
-        const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>((tasks + 63) / 64, (int64_t)c->n_cu * 16));
+        // one wave per (sequence, sample), four waves per workgroup, a persistent
+        // grid of up to 8 workgroups per CU; each wave owns a traceback stack
+        const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>((tasks + 3) / 4, (int64_t)c->n_cu * 8));
         const int cap = 2 * b->Lmax + 8;
         if (n_samples > b->smp_cap || sgrid > b->sgrid || cap > b->stack_cap) {
             region_put(c, b->r_smp);
@@ -315,12 +317,12 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
             const int g_cap = std::max(sgrid, b->sgrid);
             auto lay = [&](Carver& k) {
                 k.take<uint8_t>((int64_t)ns_cap * b->total_chars);
-                k.take<uint32_t>((int64_t)g_cap * 64 * cap);
+                k.take<uint32_t>((int64_t)g_cap * 4 * cap);
             };
             if ((rc = region_get(c, b->r_smp, measure(lay)))) return rc;
             Carver k{(char*)b->r_smp.p};
             b->samp_out = k.take<uint8_t>((int64_t)ns_cap * b->total_chars);
-            b->stack = k.take<uint32_t>((int64_t)g_cap * 64 * cap);
+            b->stack = k.take<uint32_t>((int64_t)g_cap * 4 * cap);
             b->smp_cap = ns_cap;
             b->stack_cap = cap;
         }

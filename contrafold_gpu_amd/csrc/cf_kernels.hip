@@ -1481,32 +1481,71 @@ struct CfRng {
     }
 };
 
-// One Boltzmann traceback per lane (the reference's state machine,
-// gpu.py:220-477, with candidate order unchanged; probabilities use the
-// hardware exp2, sampling is checked statistically, not bit for bit).  Each state draws one
-// uniform r and walks its candidates' probabilities exp(w - total), where
-// total is the state's own inside value; the multiloop options are grouped
-// through the stored split FM2 and resolved with a second draw.  When FP32
-// round-off leaves r above the running sum the last candidate seen is taken.
-__global__ void __launch_bounds__(64) cf_sample_kernel(
+// Stochastic traceback, one wave per (sequence, sample): the reference's state
+// machine (gpu.py:220-477) with its candidate order, walked cooperatively.
+// Each state draws one uniform r (Philox4x32-10, keyed by the sequence's key
+// and the sample index, so a sample does not depend on its batch); the lanes
+// evaluate 64 candidates' probabilities exp(w - total) at a time, a wave
+// prefix sum gives the running sums, and the first candidate whose running
+// sum reaches r is taken (the last existing one if round-off leaves r above
+// the total).  The multiloop branch is resolved through the stored split FM2
+// with a second draw.  Sums are formed in a different order than the
+// reference's serial loop and the hardware exp2 is used, so sampling is
+// checked statistically, not bit for bit.
+__device__ __forceinline__ float cf_wave_incl_sum(float v, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const float t = __shfl_up(v, d, WAVE);
+        if (lane >= d) v = v + t;
+    }
+    return v;
+}
+
+// One chunk of a candidate scan.  `c` is the running sum before the chunk.
+// Returns the lane of the picked candidate, or -1 (then `c` has the chunk's
+// sum added and `last` the highest lane holding a candidate, if any).
+__device__ __forceinline__ int cf_scan_pick(float p, bool valid, float r, float& c, int& last, int lane) {
+    const float cum = cf_wave_incl_sum(valid ? p : 0.0f, lane);
+    const float run = c + cum;
+    const uint64_t hit = __ballot(valid && r <= run);
+    if (hit) return __ffsll((unsigned long long)hit) - 1;
+    c = c + __shfl(cum, WAVE - 1, WAVE);
+    const uint64_t any = __ballot(valid);
+    if (any) last = 63 - __clzll((long long)any);
+    return -1;
+}
+
+__global__ void __launch_bounds__(256) cf_sample_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ sidx, int nseq, int nsamp, uint64_t seed,
     const uint64_t* __restrict__ keys, const uint8_t* __restrict__ codes, const uint8_t* __restrict__ masks,
     const CfParams<float>* __restrict__ P, const float* __restrict__ FC, const float* __restrict__ FM,
     const float* __restrict__ FM1, const float* __restrict__ FM2, const float* __restrict__ F5,
     uint32_t* __restrict__ stack, int stack_cap, uint8_t* __restrict__ out, const int64_t* __restrict__ out_off) {
-    // the whole parameter block in LDS: every candidate reads several entries
     __shared__ CfParams<float> sp;
+    __shared__ uint8_t wl1[496], wl2[496];  // single-branch window order: l1 ascending, l2 ascending
     for (int t = threadIdx.x; t < CF_NTAB; t += blockDim.x) sp.t[t] = P->t[t];
     for (int t = threadIdx.x; t < 625; t += blockDim.x) sp.JB[t] = P->JB[t];
+    for (int t = threadIdx.x; t < 496; t += blockDim.x) {
+        int l1 = 0, rem = t;
+        while (rem >= 31 - l1) {
+            rem -= 31 - l1;
+            ++l1;
+        }
+        wl1[t] = (uint8_t)l1;
+        wl2[t] = (uint8_t)rem;
+    }
     __syncthreads();
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nw = (int)(blockDim.x / WAVE);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const int64_t total = (int64_t)nseq * nsamp;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t gsz = (int64_t)gridDim.x * blockDim.x;
-    uint32_t* __restrict__ st = stack + gid * stack_cap;
+    const int64_t gw = (int64_t)blockIdx.x * nw + wv;
+    const int64_t gws = (int64_t)gridDim.x * nw;
+    uint32_t* __restrict__ st = stack + gw * stack_cap;
     const float* __restrict__ T = sp.t;
     const float* __restrict__ JBt = sp.JB;
     const float mb = T[CF_O_MB], mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
-    for (int64_t task = gid; task < total; task += gsz) {
+    for (int64_t task = gw; task < total; task += gws) {
         const int li = (int)(task / nsamp);
         const int smp = (int)(task % nsamp);
         const int b = sidx[li];
@@ -1523,7 +1562,7 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
         const float* __restrict__ fm2_t = FM2 + sd.toff;
         const float* __restrict__ f5 = F5 + sd.coff;
         uint8_t* __restrict__ o = out + out_off[b] + (int64_t)smp * L;
-        for (int q = 0; q < L; ++q) o[q] = '.';
+        for (int q = lane; q < L; q += WAVE) o[q] = '.';
         CfRng rng;
         const uint64_t key = keys ? keys[b] : (uint64_t)b;
         rng.key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32) ^ (uint32_t)(key >> 32));
@@ -1531,14 +1570,36 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
         rng.sample = (uint32_t)smp;
         rng.ctr = 0;
         rng.have = 0;
-        int sp = 0;
+        // the stack is wave-uniform: every lane stores the same entry and reads back its own store
+        int sp_ = 0;
         auto push = [&](int typ, int i, int j) {
-            if (sp < stack_cap) st[sp++] = (uint32_t)typ | ((uint32_t)i << 2) | ((uint32_t)j << 17);
+            if (sp_ < stack_cap) st[sp_++] = (uint32_t)typ | ((uint32_t)i << 2) | ((uint32_t)j << 17);
+        };
+        // multiloop split of FM2[i][j]: k with weight FM1[i][k] + FM[k][j]
+        auto split = [&](int i, int j, float fm2) -> int {
+            const float r2 = rng.next();
+            float c2 = 0.0f;
+            int last = -1;
+            for (int k0 = i + 1; k0 < j; k0 += WAVE) {
+                const int k = k0 + lane;
+                float p = 0.0f;
+                bool valid = false;
+                if (k < j) {
+                    const float x = fm1_t[cf_tri32(L, i, k)], y = fm_t[cf_tri32(L, k, j)];
+                    valid = x > CF_HALF && y > CF_HALF;
+                    p = __expf(x + y - fm2);
+                }
+                int lastc = -1;
+                const int t = cf_scan_pick(p, valid, r2, c2, lastc, lane);
+                if (t >= 0) return k0 + t;
+                if (lastc >= 0) last = k0 + lastc;
+            }
+            return last;
         };
         push(0, 0, L);
-        while (sp > 0) {
-            --sp;
-            const uint32_t e0 = st[sp];
+        while (sp_ > 0) {
+            --sp_;
+            const uint32_t e0 = st[sp_];
             const int typ = e0 & 3, i = (e0 >> 2) & 0x7fff, j = (e0 >> 17) & 0x7fff;
             if (typ == 0) {  // F5(j): j unpaired, or pair (k+1, j) closing an exterior helix
                 if (j == 0) continue;
@@ -1548,19 +1609,32 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                 int pick = -1;
                 if (!(r <= c)) {
                     const int sj = s[j];
-                    for (int k = 0; k < j; ++k) {
-                        if (!allow(k + 1, j)) continue;
-                        const float fc = fc_t[cf_tri32(L, k + 1, j - 1)];
-                        if (!(fc > CF_HALF) || !(f5[k] > CF_HALF)) continue;
-                        const int sk1 = s[k + 1];
-                        float jA = T[CF_O_HC + cf_i2(sj, sk1)];
-                        if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s[j + 1])];
-                        if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s[k])];
-                        const float w = f5[k] + fc + ep + T[CF_O_BP + cf_i2(sk1, sj)] + jA;
-                        c += __expf(w - tot);
-                        pick = k;
-                        if (r <= c) break;
+                    int last = -1;
+                    for (int k0 = 0; k0 < j; k0 += WAVE) {
+                        const int k = k0 + lane;
+                        float p = 0.0f;
+                        bool valid = false;
+                        if (k < j && allow(k + 1, j)) {
+                            const float fc = fc_t[cf_tri32(L, k + 1, j - 1)];
+                            if (fc > CF_HALF && f5[k] > CF_HALF) {
+                                const int sk1 = s[k + 1];
+                                float jA = T[CF_O_HC + cf_i2(sj, sk1)];
+                                if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s[j + 1])];
+                                if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s[k])];
+                                const float w = f5[k] + fc + ep + T[CF_O_BP + cf_i2(sk1, sj)] + jA;
+                                p = __expf(w - tot);
+                                valid = true;
+                            }
+                        }
+                        int lastc = -1;
+                        const int t = cf_scan_pick(p, valid, r, c, lastc, lane);
+                        if (t >= 0) {
+                            pick = k0 + t;
+                            break;
+                        }
+                        if (lastc >= 0) last = k0 + lastc;
                     }
+                    if (pick < 0) pick = last;
                 }
                 if (pick < 0) {
                     push(0, 0, j - 1);
@@ -1578,64 +1652,62 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                 const float r = rng.next();
                 float c = __expf(jB + T[CF_O_HP + (d <= 30 ? d : 30)] - tot);
                 if (r <= c) continue;  // hairpin
-                int pp = -1, pq = -1;
-                const int pmax = min(i + CF_CMAX, j);
-                bool done = false;
-                for (int p = i; p <= pmax && !done; ++p) {
-                    const int l1 = p - i;
-                    const int qmin = max(p + 2, p - i + j - CF_CMAX);
-                    for (int q = j; q >= qmin; --q) {
-                        if (!allow(p + 1, q)) continue;
-                        const float fc = fc_t[cf_tri32(L, p + 1, q - 1)];
-                        if (!(fc > CF_HALF)) continue;
-                        const int l2 = j - q;
-                        float e;
-                        if (l1 == 0 && l2 == 0) {
-                            e = T[CF_O_BP + cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
-                        } else {
-                            float sn = 0.0f;
-                            if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
-                            else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
-                            else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
-                            e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s[p + 1], s[q])] + jB +
-                                JBt[cf_i4(s[q], s[p + 1], s[q + 1], s[p])] + sn;
+                // single-branch window: rows l1 = 0 .. min(30, d-2), q = j - l2 >= p + 2
+                const int nrows = min(CF_CMAX, d - 2) + 1;
+                const int tend = nrows > 0 ? 31 * nrows - nrows * (nrows - 1) / 2 : 0;
+                int pick = -1, last = -1;
+                for (int t0 = 0; t0 < tend; t0 += WAVE) {
+                    const int t = t0 + lane;
+                    float p = 0.0f;
+                    bool valid = false;
+                    if (t < tend) {
+                        const int l1 = wl1[t], l2 = wl2[t];
+                        const int pp = i + l1, q = j - l2;
+                        if (q >= pp + 2 && allow(pp + 1, q)) {
+                            const float fc = fc_t[cf_tri32(L, pp + 1, q - 1)];
+                            if (fc > CF_HALF) {
+                                float e;
+                                if (l1 == 0 && l2 == 0) {
+                                    e = T[CF_O_BP + cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
+                                } else {
+                                    float sn = 0.0f;
+                                    if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
+                                    else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
+                                    else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
+                                    e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s[pp + 1], s[q])] + jB +
+                                        JBt[cf_i4(s[q], s[pp + 1], s[q + 1], s[pp])] + sn;
+                                }
+                                p = __expf(e + fc - tot);
+                                valid = true;
+                            }
                         }
-                        c += __expf(e + fc - tot);
-                        pp = p;
-                        pq = q;
-                        if (r <= c) { done = true; break; }
                     }
+                    int lastc = -1;
+                    const int tp = cf_scan_pick(p, valid, r, c, lastc, lane);
+                    if (tp >= 0) {
+                        pick = t0 + tp;
+                        break;
+                    }
+                    if (lastc >= 0) last = t0 + lastc;
                 }
-                const float fm2 = fm2_t[cf_tri32(L, i, j)];
-                if (!done && fm2 > CF_HALF) {
-                    float jA = T[CF_O_HC + cf_i2(si, sj1)];
-                    if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
-                    if (j > 0) jA = jA + T[CF_O_DR + cf_i3(si, sj1, sj)];
-                    const float wm = fm2 + jA + mp + mb;
-                    c += __expf(wm - tot);
-                    {  // chosen by r, or as the last candidate when round-off left r above the sum
-                        // multiloop: choose the split k with weight FM1[i][k] + FM[k][j] over FM2[i][j]
-                        const float r2 = rng.next();
-                        float c2 = 0.0f;
-                        int kk = -1;
-                        for (int k = i + 1; k < j; ++k) {
-                            const float x = fm1_t[cf_tri32(L, i, k)], y = fm_t[cf_tri32(L, k, j)];
-                            if (!(x > CF_HALF && y > CF_HALF)) continue;
-                            c2 += __expf(x + y - fm2);
-                            kk = k;
-                            if (r2 <= c2) break;
-                        }
+                if (pick < 0) {
+                    const float fm2 = fm2_t[cf_tri32(L, i, j)];
+                    if (fm2 > CF_HALF) {
+                        // multiloop: taken when r passed every single-branch candidate
+                        const int kk = split(i, j, fm2);
                         if (kk >= 0) {
                             push(3, i, kk);
                             push(2, kk, j);
                             continue;
                         }
                     }
+                    pick = last;
                 }
-                if (pp >= 0) {
+                if (pick >= 0) {
+                    const int pp = i + wl1[pick], q = j - wl2[pick];
                     o[pp] = '(';
-                    o[pq - 1] = ')';
-                    push(1, pp + 1, pq - 1);
+                    o[q - 1] = ')';
+                    push(1, pp + 1, q - 1);
                 }
             } else if (typ == 2) {  // FM(i, j): split, j unpaired, or one branch (in that order)
                 const float tot = fm_t[cf_tri32(L, i, j)];
@@ -1650,16 +1722,7 @@ __global__ void __launch_bounds__(64) cf_sample_kernel(
                 if (opt < 0 && hu) { c += __expf(wun + mu - tot); if (r <= c || !h1) opt = 1; }
                 if (opt < 0 && h1) opt = 2;
                 if (opt == 0) {
-                    const float r2 = rng.next();
-                    float c2 = 0.0f;
-                    int kk = -1;
-                    for (int k = i + 1; k < j; ++k) {
-                        const float x = fm1_t[cf_tri32(L, i, k)], y = fm_t[cf_tri32(L, k, j)];
-                        if (!(x > CF_HALF && y > CF_HALF)) continue;
-                        c2 += __expf(x + y - fm2);
-                        kk = k;
-                        if (r2 <= c2) break;
-                    }
+                    const int kk = split(i, j, fm2);
                     if (kk >= 0) { push(3, i, kk); push(2, kk, j); }
                 } else if (opt == 1) {
                     push(2, i, j - 1);
