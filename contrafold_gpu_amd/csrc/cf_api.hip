@@ -118,11 +118,11 @@ int launch_inside(cf_batch* b) {
         const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, G);
         if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the inside kernel's LDS plan");
         if (pack)
-            hipLaunchKernelGGL((cf_inside_kernel<false, true>), dim3((nl + G - 1) / G), dim3(64 * G), plan.total,
+            hipLaunchKernelGGL((cf_inside_kernel<true>), dim3((nl + G - 1) / G), dim3(64 * G), plan.total,
                                b->stream, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC,
                                b->FM, b->FM1, b->FM2, b->FCX, bk.Lmax);
         else
-            hipLaunchKernelGGL((cf_inside_kernel<false, false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
+            hipLaunchKernelGGL((cf_inside_kernel<false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
                                b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
                                b->FM1, b->FM2, b->FCX, bk.Lmax);
         CF_HIP(c, hipGetLastError(), "inside launch");
@@ -652,7 +652,7 @@ int cf_create(int device, cf_ctx** out) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
         c->lds_max = v;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0) c->n_cu = v;
-    const void* dyn[] = {(const void*)cf_inside_kernel<false, false>, (const void*)cf_inside_kernel<false, true>,
+    const void* dyn[] = {(const void*)cf_inside_kernel<false>, (const void*)cf_inside_kernel<true>,
                          (const void*)cf_outside_kernel<true, false>, (const void*)cf_outside_kernel<false, false>,
                          (const void*)cf_outside_kernel<false, true>, (const void*)cf_post_kernel,
                          (const void*)cf_exterior_kernel,

@@ -16,7 +16,7 @@
 #define CF_NEG64 (-1e30)
 #define CF_HALF64 (-5e29)
 #define CF_CMAX 30
-#define CF_RING 33  // FC diagonals kept in LDS: the window reads s-2 .. s-32
+#define CF_RING 34  // FCo diagonals kept in LDS: two diagonals written, d+2 .. d+32 read
 
 // Flat parameter-table layout shared with the host (2577 values; the order
 // of the dict returned by cpu.load, cpu.py:132-135).

@@ -63,7 +63,7 @@ __host__ __device__ inline CfLds cf_lds_plan(int Lmax, bool ring, int dbuf_elem,
     o = cf_align(o, 16);
     p.o_dbuf = o; o += (Lmax + 1 > 128 ? Lmax + 1 : 128) * dbuf_elem;  // also the posterior queue
     o = cf_align(o, 16);
-    p.o_alist = o; o += (Lmax + 1) * 2;
+    p.o_alist = o; o += 2 * (Lmax + 1) * 2;  // two diagonals' pairable cells
     o = cf_align(o, 16);
     p.o_cnt = o; o += 16;
     p.o_ring = o;
@@ -216,7 +216,18 @@ __device__ __forceinline__ bool cf_allow(const CfShared& S, int a, int b) {
 // FP32 inside sweep (cpu.py:595-665; gpu.py:79-168).  Writes FC, FM1, FM,
 // FM2 (the multiloop split, kept for sampling and gradients) and the exterior
 // copy FCX of FC.  F5 is produced by cf_exterior_kernel.
-template <bool RING, bool PACK>
+//
+// Diagonals are taken two at a time.  FM2 and FC of diagonal d read FM1 / FM
+// / FC of diagonals <= d-2 only (and FC reads FM2 of its own cell), so FM2 and
+// FC of diagonals s and s+1 are computed together -- twice the cells per pass
+// over the team's lanes -- while FM1 and FM, which read diagonal d-1, follow
+// one diagonal at a time:
+//   A: FM2 of every cell of s and s+1; pairable cells of both into a list
+//   B: FC of the listed cells
+//   C: FM1, FM of diagonal s       D: FM1, FM of diagonal s+1
+// Each cell's terms are folded in the reference's order, so every table is
+// bit-identical to a one-diagonal-at-a-time sweep.
+template <bool PACK>
 __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
                                                         int nseq, const uint8_t* __restrict__ codes,
                                                         const uint8_t* __restrict__ masks,
@@ -226,7 +237,7 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
     extern __shared__ __align__(16) unsigned char smem[];
     const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
     const CfTeam tm = cf_team<PACK>(G);
-    const CfLds plan = cf_lds_plan(Lmax, RING, 4, G);
+    const CfLds plan = cf_lds_plan(Lmax, false, 4, G);
     const CfShared S = cf_carve(smem, plan, tm.team);
     cf_stage_params(S, P);
     __syncthreads();
@@ -238,9 +249,6 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
 
     const float mb = P->t[CF_O_MB], mu = P->t[CF_O_MU], mp = P->t[CF_O_MP];
     const float* __restrict__ T = P->t;
-    float* __restrict__ ring = (float*)S.ring;
-    float* __restrict__ fm2d = (float*)S.dbuf;
-    const int RS = Lmax + 1;
     float* __restrict__ fc_t = FC + sd.toff;
     float* __restrict__ fm_t = FM + sd.toff;
     float* __restrict__ fm1_t = FM1 + sd.toff;
@@ -248,91 +256,72 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
     float* __restrict__ fcx_t = FCX + sd.xoff;
     const float4* __restrict__ C = S.lse;
 
-    for (int s = 0; s <= L; ++s) {
-        const int ncell = L + 1 - s;
-        const int64_t db = cf_dbase(L, s);
-        const int slot = s % CF_RING;
-        // ---- phase 1: multiloop split, FM1, FM for every cell; collect pairable cells
-        for (int i = tid; i < ncell; i += NT) {
-            const int j = i + s;
+    for (int s = 0; s <= L; s += 2) {
+        const int step = s >> 1;
+        const int n0 = L + 1 - s;
+        const int n1 = (s + 1 <= L) ? L - s : 0;
+        // ---- A: multiloop split FM2 of diagonals s and s+1; collect pairable cells
+        for (int c = tid; c < n0 + n1; c += NT) {
+            const int hi = (c >= n0) ? 1 : 0;
+            const int d = s + hi;
+            const int i = c - (hi ? n0 : 0);
+            const int j = i + d;
             const bool inner = (i > 0) && (j < L);
             const bool alw = inner && cf_allow(S, i, j + 1);
+            const int db = S.dtab[d];
             float fm2 = CF_NEG;
-            if (inner && s >= 4) {
+            if (inner && d >= 4) {
                 // FM2[i][j] = LSE_{k=i+2..j-2} FM1[i][k] + FM[k][j]  (k = i + t).
                 // Loads of 8 consecutive terms are issued before the 8 folds,
                 // so memory latency overlaps; the fold order is unchanged.
-                int64_t ia = cf_dbase(L, 2) + i;
-                int64_t ib = cf_dbase(L, s - 2) + i + 2;
+                int ia = S.dtab[2] + i;
+                int ib = S.dtab[d - 2] + i + 2;
                 int t = 2;
-                for (; t + 7 <= s - 2; t += 8) {
+                for (; t + 7 <= d - 2; t += 8) {
                     float x[8], y[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         x[u] = fm1_t[ia];
                         y[u] = fm_t[ib];
                         ia += L + 1 - (t + u);
-                        ib -= L + 1 - (s - (t + u) - 1);
+                        ib -= L + 1 - (d - (t + u) - 1);
                         ib += 1;
                     }
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
                         fm2 = cf_lse(fm2, (x[u] > CF_HALF && y[u] > CF_HALF) ? x[u] + y[u] : CF_NEG, C);
                 }
-                for (; t <= s - 2; ++t) {
+                for (; t <= d - 2; ++t) {
                     const float x = fm1_t[ia];
                     const float y = fm_t[ib];
                     fm2 = cf_lse(fm2, (x > CF_HALF && y > CF_HALF) ? x + y : CF_NEG, C);
                     ia += L + 1 - t;
-                    ib -= L + 1 - (s - t - 1);
+                    ib -= L + 1 - (d - t - 1);
                     ib += 1;
                 }
             }
-            float fm1 = CF_NEG, fmv = CF_NEG;
-            if (inner && s >= 2) {
-                const int si1 = S.s8[i + 1], sj = S.s8[j];
-                if (cf_allow(S, i + 1, j)) {
-                    const float fcin = RING ? ring[((s - 2) % CF_RING) * RS + i + 1] : fc_t[cf_dbase(L, s - 2) + i + 1];
-                    if (fcin > CF_HALF) {
-                        float jA = T[CF_O_HC + cf_i2(sj, si1)];
-                        if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, S.s8[j + 1])];
-                        if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, S.s8[i])];
-                        float v = fcin + jA;
-                        v = v + mp;
-                        v = v + S.bp[cf_i2(si1, sj)];
-                        fm1 = v;
-                    }
-                }
-                const float f1n = fm1_t[cf_dbase(L, s - 1) + i + 1];
-                if (f1n > CF_HALF) fm1 = cf_lse(fm1, f1n + mu, C);
-                if (fm2 > CF_HALF) fmv = fm2;
-                const float fml = fm_t[cf_dbase(L, s - 1) + i];
-                if (fml > CF_HALF) fmv = cf_lse(fmv, fml + mu, C);
-                if (fm1 > CF_HALF) fmv = cf_lse(fmv, fm1, C);
-            }
-            fm1_t[db + i] = fm1;
-            fm_t[db + i] = fmv;
             fm2_t[db + i] = fm2;
             if (alw) {
-                fm2d[i] = fm2;
-                const int slotn = atomicAdd(&S.cnt[s & 1], 1);
-                S.alist[slotn] = (uint16_t)i;
+                const int slotn = atomicAdd(&S.cnt[step & 1], 1);
+                S.alist[slotn] = (uint16_t)(i | (hi << 15));
             } else {
                 fc_t[db + i] = CF_NEG;
-                if (RING) ring[slot * RS + i] = CF_NEG;
                 if (inner) fcx_t[cf_xbase(j + 1) + i - 1] = CF_NEG;
             }
         }
         cf_team_sync<PACK>();
-        // ---- phase 2: FC for the pairable cells (hairpin, single-branch window, multiloop)
-        const int nal = S.cnt[s & 1];
-        if (tid == 0) S.cnt[(s + 1) & 1] = 0;
+        // ---- B: FC of the pairable cells (hairpin, single-branch window, multiloop)
+        const int nal = S.cnt[step & 1];
+        if (tid == 0) S.cnt[(step + 1) & 1] = 0;
         for (int idx = tid; idx < nal; idx += NT) {
-            const int i = S.alist[idx];
-            const int j = i + s;
+            const int e0 = S.alist[idx];
+            const int i = e0 & 0x7fff;
+            const int d = s + (e0 >> 15);
+            const int j = i + d;
+            const int db = S.dtab[d];
             const int si = S.s8[i], sj1 = S.s8[j + 1], si1 = S.s8[i + 1], sj = S.s8[j];
             const float jB = S.JB[cf_i4(si, sj1, si1, sj)];
-            float acc = jB + T[CF_O_HP + (s < 30 ? s : 30)];
+            float acc = jB + T[CF_O_HP + (d < 30 ? d : 30)];
             const float e_stack = S.bp[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
             const float sn01 = T[CF_O_B01 + sj], sn10 = T[CF_O_B01 + si1], sn11 = T[CF_O_I11 + cf_i2(si1, sj)];
             const int pend = min(i + CF_CMAX, j - 2);
@@ -344,7 +333,7 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
                 bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
             }
             // Candidates (p ascending, q descending) are gathered 8 at a time
-            // -- their LDS reads do not depend on acc -- then folded in order;
+            // -- their reads do not depend on acc -- then folded in order;
             // exhausted slots hold NEG, which leaves acc unchanged.
             bool more = (p <= pend);
             while (more) {
@@ -363,8 +352,8 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
                         bits &= ~(1u << k);
                         const int q = qlo + k;
                         const int l1 = p - i, l2 = j - q;
-                        const int dd = s - l1 - l2 - 2;
-                        const float fc = RING ? ring[(dd % CF_RING) * RS + p + 1] : fc_t[S.dtab[dd] + p + 1];
+                        const int dd = d - l1 - l2 - 2;
+                        const float fc = fc_t[S.dtab[dd] + p + 1];
                         const int qi = S.qinfo[q];
                         const float2 bj = S.BJ[(qi & 0xffff) + rowJ];
                         float e = S.cs[l1 * 31 + l2] + bj.y;
@@ -383,7 +372,7 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
 #pragma unroll
                 for (int u = 0; u < 8; ++u) acc = cf_lse(acc, v[u], C);
             }
-            const float fm2 = fm2d[i];
+            const float fm2 = fm2_t[db + i];
             if (fm2 > CF_HALF) {
                 float jA = T[CF_O_HC + cf_i2(si, sj1)];
                 if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
@@ -394,10 +383,44 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
                 acc = cf_lse(acc, v, C);
             }
             fc_t[db + i] = acc;
-            if (RING) ring[slot * RS + i] = acc;
             fcx_t[cf_xbase(j + 1) + i - 1] = acc;
         }
         cf_team_sync<PACK>();
+        // ---- C, D: FM1 and FM of diagonal s, then of s+1 (each reads its predecessor)
+        for (int d = s; d <= s + 1 && d <= L; ++d) {
+            const int ncell = L + 1 - d;
+            const int db = S.dtab[d];
+            for (int i = tid; i < ncell; i += NT) {
+                const int j = i + d;
+                const bool inner = (i > 0) && (j < L);
+                float fm1 = CF_NEG, fmv = CF_NEG;
+                if (inner && d >= 2) {
+                    const int si1 = S.s8[i + 1], sj = S.s8[j];
+                    if (cf_allow(S, i + 1, j)) {
+                        const float fcin = fc_t[S.dtab[d - 2] + i + 1];
+                        if (fcin > CF_HALF) {
+                            float jA = T[CF_O_HC + cf_i2(sj, si1)];
+                            if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, S.s8[j + 1])];
+                            if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, S.s8[i])];
+                            float v = fcin + jA;
+                            v = v + mp;
+                            v = v + S.bp[cf_i2(si1, sj)];
+                            fm1 = v;
+                        }
+                    }
+                    const float f1n = fm1_t[S.dtab[d - 1] + i + 1];
+                    if (f1n > CF_HALF) fm1 = cf_lse(fm1, f1n + mu, C);
+                    const float fm2 = fm2_t[db + i];
+                    if (fm2 > CF_HALF) fmv = fm2;
+                    const float fml = fm_t[S.dtab[d - 1] + i];
+                    if (fml > CF_HALF) fmv = cf_lse(fmv, fml + mu, C);
+                    if (fm1 > CF_HALF) fmv = cf_lse(fmv, fm1, C);
+                }
+                fm1_t[db + i] = fm1;
+                fm_t[db + i] = fmv;
+            }
+            cf_team_sync<PACK>();
+        }
     }
 }
 
@@ -557,13 +580,22 @@ __global__ void __launch_bounds__(256) cf_exterior_kernel(const CfSeq* __restric
 }
 
 // =================================================================== outside
-// FP32 outside sweep + posterior (cpu.py:680-819), as per-cell gathers that
-// replay the reference's scatter order.  Diagonals are walked from s = L
-// down to 0; each target cell folds its sources (all on larger diagonals) in
-// the order the reference's (i ascending, j descending) sweep delivers them.
-// POST of the pair (a, b+1) is gathered in the same step as FCo[a][b]: its
-// single-branch sources are exactly FCo's, visited in reverse order.
-// POSTF is indexed by FC cell: POSTF[tri(a, b)] = P(a pairs b+1).
+// FP32 outside sweep (cpu.py:680-758) as per-cell gathers that replay the
+// reference's scatter order: each target cell folds its sources (all on larger
+// diagonals) in the order the reference's (i ascending, j descending) sweep
+// delivers them.  The posterior is a separate kernel (cf_post_kernel).
+//
+// Diagonals are taken two at a time, s and s-1 (descending).  FCo of
+// diagonal d reads FCo of diagonals >= d+2 and FM1o of d+2; the split part of
+// FMo reads FM2o of diagonals >= d+2; so those are computed for both
+// diagonals together (twice the cells per pass).  FMo's last term reads FMo of
+// d+1, and FM2o / FM1o read FMo, FCo and FM1o of d+1 on the same row, so they
+// follow one diagonal at a time:
+//   A: split part of FMo for s and s-1; pairable cells of both into a list
+//   B: FCo of the listed cells; FMo of s completed
+//   C: FMo of s-1 completed; FM2o, FM1o of s       D: FM2o, FM1o of s-1
+// Each cell's terms are folded in the reference's order, so every table is
+// bit-identical to a one-diagonal-at-a-time sweep.
 template <bool RING, bool PACK>
 __global__ void __launch_bounds__(256) cf_outside_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ order, int nseq, const uint8_t* __restrict__ codes,
@@ -588,7 +620,6 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
     const float mb = P->t[CF_O_MB], mu = P->t[CF_O_MU], mp = P->t[CF_O_MP], ep = P->t[CF_O_EP];
     const float* __restrict__ T = P->t;
     float* __restrict__ ring = (float*)S.ring;
-    float* __restrict__ fmod = (float*)S.dbuf;
     const int RS = Lmax + 1;
     const float* __restrict__ fc_t = FC + sd.toff;
     const float* __restrict__ fm_t = FM + sd.toff;
@@ -600,29 +631,115 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
     float* __restrict__ fm1o_t = FM1o + sd.toff;
     float* __restrict__ fm2o_t = FM2o + sd.toff;
     const float4* __restrict__ C = S.lse;
-    const float4* __restrict__ E = S.ex;
-    const float Z = f5[L];
 
-    for (int s = L; s >= 0; --s) {
-        const int ncell = L + 1 - s;
-        const int64_t db = cf_dbase(L, s);
-        const int slot = s % CF_RING;
-        // ---- phase 1: FMo for every multiloop cell; collect pairable cells
+    // FMo[a][b] += FMo[a][b+1] + mu: the last source of a completed FMo cell
+    auto fmo_finish = [&](int d) {
+        const int ncell = L + 1 - d;
+        const int db = S.dtab[d];
+        for (int a = tid; a < ncell; a += NT) {
+            const int b = a + d;
+            const bool dom = (a > 0) && (a + 2 <= b) && (b < L);
+            if (dom && b + 1 < L) {
+                const float x = fmo_t[S.dtab[d + 1] + a];
+                if (x > CF_HALF) fmo_t[db + a] = cf_lse(fmo_t[db + a], x + mu, C);
+            }
+        }
+    };
+    // FM2o and FM1o of every multiloop cell of diagonal d
+    auto fm12o = [&](int d) {
+        const int ncell = L + 1 - d;
+        const int db = S.dtab[d];
         for (int i0 = __builtin_amdgcn_readfirstlane(tid - lane); i0 < ncell; i0 += NT) {
             const int a = i0 + lane;
-            const int b = a + s;
+            const int b = a + d;
             const bool live = a < ncell;
+            const bool dom = live && (a > 0) && (a + 2 <= b) && (b < L);
+            float fm2o = CF_NEG, f1 = CF_NEG, fmo = CF_NEG;
+            if (dom) {
+                fmo = fmo_t[db + a];
+                fm2o = fmo;
+                if (cf_allow(S, a, b + 1)) {
+                    const float fco = RING ? ring[(d % CF_RING) * RS + a] : fco_t[db + a];
+                    if (fco > CF_HALF) {
+                        const int sa = S.s8[a], sb1 = S.s8[b + 1];
+                        float jA = T[CF_O_HC + cf_i2(sa, sb1)];
+                        if (a < L) jA = jA + T[CF_O_DL + cf_i3(sa, sb1, S.s8[a + 1])];
+                        if (b > 0) jA = jA + T[CF_O_DR + cf_i3(sa, sb1, S.s8[b])];
+                        float v = fco + jA;
+                        v = v + mp;
+                        v = v + mb;
+                        fm2o = cf_lse(fm2o, v, C);
+                    }
+                }
+                // (i) FM1o[a-1][b] + mu
+                if (a - 1 > 0 && a + 1 <= b) {
+                    const float x = fm1o_t[S.dtab[d + 1] + a - 1];
+                    if (x > CF_HALF) f1 = cf_lse(f1, x + mu, C);
+                }
+            }
+            // (ii) FM2o(a, b+v) + FM[b][b+v] for v = L-1-b .. 2 descending (lanes step v together)
+            const int vmax = L - 1 - (i0 + d);
+            int v = vmax;
+            int ox = (int)cf_dbase(L, d + v), oy = (int)cf_dbase(L, v);  // diagonals d + v (FM2o) and v (FM)
+            for (; v - 7 >= 2; v -= 8) {
+                float x[8], y[8];
+                bool g[8];
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    const int vv = v - w;
+                    g[w] = dom & (vv <= L - 1 - b);
+                    x[w] = fm2o_t[ox + (g[w] ? a : 0)];
+                    y[w] = fm_t[oy + (g[w] ? b : 0)];
+                    ox -= L + 2 - (d + vv);
+                    oy -= L + 2 - vv;
+                }
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    const bool ok = g[w] & (x[w] > CF_HALF) & (y[w] > CF_HALF);
+                    f1 = cf_lse(f1, ok ? x[w] + y[w] : CF_NEG, C);
+                }
+            }
+            for (; v >= 2; --v) {
+                const bool g = dom & (v <= L - 1 - b);
+                const float x = fm2o_t[ox + (g ? a : 0)];
+                const float y = fm_t[oy + (g ? b : 0)];
+                ox -= L + 2 - (d + v);
+                oy -= L + 2 - v;
+                const bool ok = g & (x > CF_HALF) & (y > CF_HALF);
+                f1 = cf_lse(f1, ok ? x + y : CF_NEG, C);
+            }
+            if (dom && fmo > CF_HALF) f1 = cf_lse(f1, fmo, C);
+            if (live) {
+                fm2o_t[db + a] = fm2o;
+                fm1o_t[db + a] = f1;
+            }
+        }
+    };
+
+    for (int s = L; s >= 0; s -= 2) {
+        const int step = (L - s) >> 1;
+        const int n0 = L + 1 - s;
+        const int n1 = (s >= 1) ? L + 2 - s : 0;  // cells of diagonal s-1
+        // ---- A: split part of FMo for diagonals s and s-1; collect pairable cells
+        for (int c0 = __builtin_amdgcn_readfirstlane(tid - lane); c0 < n0 + n1; c0 += NT) {
+            const int c = c0 + lane;
+            const bool live = c < n0 + n1;
+            const int hi = (c >= n0) ? 1 : 0;
+            const int d = s - hi;
+            const int a = c - (hi ? n0 : 0);
+            const int b = a + d;
             const bool dom = live && (a > 0) && (a + 2 <= b) && (b < L);
             const bool alw = live && (a > 0) && (b < L) && cf_allow(S, a, b + 1);
             float fmo = CF_NEG;
-            // (i) FM2o(i', b) + FM1[i'][a] for i' = 1 .. a-2 ascending, i.e. u = a - i' descending.
-            // Lanes of a wave step u together so both reads stay contiguous.
-            const int amax = min(i0 + WAVE - 1, ncell - 1);
+            // FM2o(i', b) + FM1[i'][a] for i' = 1 .. a-2 ascending, i.e. u = a - i'
+            // descending from the pass's largest a.  Wave-uniform offsets of the
+            // diagonals d + u (FM2o, one sequence per diagonal of the pair) and
+            // u (FM1) step down with u; lanes outside the domain read cell 0
+            // of the diagonal and fold NEG, which leaves the sum unchanged.
+            const int cl = c0 + WAVE - 1;
+            const int amax = max(c0 < n0 ? min(cl, n0 - 1) : -1, cl >= n0 ? min(cl - n0, n1 - 1) : -1);
             int u = amax - 1;
-            // wave-uniform offsets of diagonals s + u (FM2o) and u (FM1), stepped
-            // down with u; lanes outside the domain read cell 0 of the diagonal
-            // and fold NEG, which leaves the sum unchanged
-            int ox = (int)cf_dbase(L, s + u), oy = (int)cf_dbase(L, u);
+            int ox0 = (int)cf_dbase(L, s + u), ox1 = (int)cf_dbase(L, s - 1 + u), oy = (int)cf_dbase(L, u);
             for (; u - 7 >= 2; u -= 8) {
                 float x[8], y[8];
                 bool g[8];
@@ -631,9 +748,10 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
                     const int uu = u - w;
                     g[w] = dom & (uu <= a - 1);
                     const int off = g[w] ? a - uu : 0;
-                    x[w] = fm2o_t[ox + off];
+                    x[w] = fm2o_t[(hi ? ox1 : ox0) + off];
                     y[w] = fm1_t[oy + off];
-                    ox -= L + 2 - (s + uu);
+                    ox0 -= L + 2 - (s + uu);
+                    ox1 -= L + 2 - (s - 1 + uu);
                     oy -= L + 2 - uu;
                 }
 #pragma unroll
@@ -645,37 +763,36 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             for (; u >= 2; --u) {
                 const bool g = dom & (u <= a - 1);
                 const int off = g ? a - u : 0;
-                const float x = fm2o_t[ox + off];
+                const float x = fm2o_t[(hi ? ox1 : ox0) + off];
                 const float y = fm1_t[oy + off];
-                ox -= L + 2 - (s + u);
+                ox0 -= L + 2 - (s + u);
+                ox1 -= L + 2 - (s - 1 + u);
                 oy -= L + 2 - u;
                 const bool ok = g & (x > CF_HALF) & (y > CF_HALF);
                 fmo = cf_lse(fmo, ok ? x + y : CF_NEG, C);
             }
-            // (ii) FMo[a][b+1] + mu
-            if (dom && b + 1 < L) {
-                const float x = fmo_t[cf_dbase(L, s + 1) + a];
-                if (x > CF_HALF) fmo = cf_lse(fmo, x + mu, C);
-            }
             if (live) {
+                const int db = S.dtab[d];
                 fmo_t[db + a] = fmo;
-                fmod[a] = fmo;
                 if (alw) {
-                    const int slotn = atomicAdd(&S.cnt[s & 1], 1);
-                    S.alist[slotn] = (uint16_t)a;
+                    const int slotn = atomicAdd(&S.cnt[step & 1], 1);
+                    S.alist[slotn] = (uint16_t)(a | (hi << 15));
                 } else {
                     fco_t[db + a] = CF_NEG;
-                    if (RING) ring[slot * RS + a] = CF_NEG;
+                    if (RING) ring[(d % CF_RING) * RS + a] = CF_NEG;
                 }
             }
         }
         cf_team_sync<PACK>();
-        // ---- phase 2: FCo for pairable cells
-        const int nal = S.cnt[s & 1];
-        if (tid == 0) S.cnt[(s + 1) & 1] = 0;
+        // ---- B: FCo of the pairable cells of both diagonals; FMo of s completed
+        const int nal = S.cnt[step & 1];
+        if (tid == 0) S.cnt[(step + 1) & 1] = 0;
         for (int idx = tid; idx < nal; idx += NT) {
-            const int a = S.alist[idx];
-            const int b = a + s;
+            const int e0 = S.alist[idx];
+            const int a = e0 & 0x7fff;
+            const int d = s - (e0 >> 15);
+            const int b = a + d;
+            const int db = S.dtab[d];
             const int sa = S.s8[a], sb1 = S.s8[b + 1], sam = S.s8[a - 1], sb2 = S.s8[b + 2];
             const float bpab = S.bp[cf_i2(sa, sb1)];
             // jA of branch (a, b+1) seen from the enclosing loop (exterior or multiloop)
@@ -687,7 +804,7 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             // bulge 0x1 / 1x0 and interior 1x1 extras of the three nearest outer pairs
             const float sn01 = T[CF_O_B01 + sb2], sn10 = T[CF_O_B01 + sam], sn11 = T[CF_O_I11 + cf_i2(sam, sb2)];
             const bool fm1dom = (a - 1 > 0) && (a + 1 <= b + 1) && (b + 1 < L);
-            const float fm1o_src = fm1dom ? fm1o_t[cf_dbase(L, s + 2) + a - 1] : CF_NEG;
+            const float fm1o_src = fm1dom ? fm1o_t[S.dtab[d + 2] + a - 1] : CF_NEG;
             const int lmax = min(CF_CMAX, a - 2);
             const int qbase = b + 2;
             float acc = CF_NEG;
@@ -708,8 +825,8 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
                 const uint32_t last = (l1 == 0) ? (bits & 1u) : 0u;
                 bits &= (l1 == 0) ? ~1u : ~0u;
                 auto term = [&](int l2) -> float {
-                    const int d = s + l1 + l2 + 2;
-                    const float fco = RING ? ring[(d % CF_RING) * RS + ip] : fco_t[S.dtab[d] + ip];
+                    const int dd = d + l1 + l2 + 2;
+                    const float fco = RING ? ring[(dd % CF_RING) * RS + ip] : fco_t[S.dtab[dd] + ip];
                     float e = S.cs[l1 * 31 + l2] + bpab;
                     e = e + S.JB[rowJ + S.qo[qbase + l2 - 1]];
                     e = e + jB2;
@@ -735,78 +852,19 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
                 }
             }
             fco_t[db + a] = acc;
-            if (RING) ring[slot * RS + a] = acc;
+            if (RING) ring[(d % CF_RING) * RS + a] = acc;
         }
+        fmo_finish(s);
         cf_team_sync<PACK>();
-        // ---- phase 3: FM2o and FM1o for every multiloop cell
-        for (int i0 = __builtin_amdgcn_readfirstlane(tid - lane); i0 < ncell; i0 += NT) {
-            const int a = i0 + lane;
-            const int b = a + s;
-            const bool live = a < ncell;
-            const bool dom = live && (a > 0) && (a + 2 <= b) && (b < L);
-            float fm2o = CF_NEG, f1 = CF_NEG;
-            if (dom) {
-                fm2o = fmod[a];
-                if (cf_allow(S, a, b + 1)) {
-                    const float fco = RING ? ring[slot * RS + a] : fco_t[db + a];
-                    if (fco > CF_HALF) {
-                        const int sa = S.s8[a], sb1 = S.s8[b + 1];
-                        float jA = T[CF_O_HC + cf_i2(sa, sb1)];
-                        if (a < L) jA = jA + T[CF_O_DL + cf_i3(sa, sb1, S.s8[a + 1])];
-                        if (b > 0) jA = jA + T[CF_O_DR + cf_i3(sa, sb1, S.s8[b])];
-                        float v = fco + jA;
-                        v = v + mp;
-                        v = v + mb;
-                        fm2o = cf_lse(fm2o, v, C);
-                    }
-                }
-                // (i) FM1o[a-1][b] + mu
-                if (a - 1 > 0 && a + 1 <= b) {
-                    const float x = fm1o_t[cf_dbase(L, s + 1) + a - 1];
-                    if (x > CF_HALF) f1 = cf_lse(f1, x + mu, C);
-                }
-            }
-            // (ii) FM2o(a, b+v) + FM[b][b+v] for v = L-1-b .. 2 descending (lanes step v together)
-            const int vmax = L - 1 - (i0 + s);
-            int v = vmax;
-            int ox = (int)cf_dbase(L, s + v), oy = (int)cf_dbase(L, v);  // diagonals s + v (FM2o) and v (FM)
-            for (; v - 7 >= 2; v -= 8) {
-                float x[8], y[8];
-                bool g[8];
-#pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    const int vv = v - w;
-                    g[w] = dom & (vv <= L - 1 - b);
-                    x[w] = fm2o_t[ox + (g[w] ? a : 0)];
-                    y[w] = fm_t[oy + (g[w] ? b : 0)];
-                    ox -= L + 2 - (s + vv);
-                    oy -= L + 2 - vv;
-                }
-#pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    const bool ok = g[w] & (x[w] > CF_HALF) & (y[w] > CF_HALF);
-                    f1 = cf_lse(f1, ok ? x[w] + y[w] : CF_NEG, C);
-                }
-            }
-            for (; v >= 2; --v) {
-                const bool g = dom & (v <= L - 1 - b);
-                const float x = fm2o_t[ox + (g ? a : 0)];
-                const float y = fm_t[oy + (g ? b : 0)];
-                ox -= L + 2 - (s + v);
-                oy -= L + 2 - v;
-                const bool ok = g & (x > CF_HALF) & (y > CF_HALF);
-                f1 = cf_lse(f1, ok ? x + y : CF_NEG, C);
-            }
-            if (dom) {
-                const float fmo = fmod[a];
-                if (fmo > CF_HALF) f1 = cf_lse(f1, fmo, C);
-            }
-            if (live) {
-                fm2o_t[db + a] = fm2o;
-                fm1o_t[db + a] = f1;
-            }
+        // ---- C: FMo of s-1 completed; FM2o, FM1o of s
+        if (s >= 1) fmo_finish(s - 1);
+        fm12o(s);
+        cf_team_sync<PACK>();
+        // ---- D: FM2o, FM1o of s-1
+        if (s >= 1) {
+            fm12o(s - 1);
+            cf_team_sync<PACK>();
         }
-        cf_team_sync<PACK>();
     }
 }
 
@@ -1914,12 +1972,10 @@ __global__ void __launch_bounds__(256) cf_grad_kernel(
 }
 
 // ------------------------------------------------------------ explicit inits
-template __global__ void cf_inside_kernel<false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
-                                                        const CfParams<float>*, float*, float*, float*, float*, float*,
-                                                        int);
-template __global__ void cf_inside_kernel<false, true>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
-                                                       const CfParams<float>*, float*, float*, float*, float*, float*,
-                                                       int);
+template __global__ void cf_inside_kernel<false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                 const CfParams<float>*, float*, float*, float*, float*, float*, int);
+template __global__ void cf_inside_kernel<true>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                const CfParams<float>*, float*, float*, float*, float*, float*, int);
 template __global__ void cf_outside_kernel<true, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                         const CfParams<float>*, const float*, const float*, const float*,
                                                         const float*, const float*, float*, float*, float*, float*,
