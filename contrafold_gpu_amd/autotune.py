@@ -18,8 +18,11 @@ CANDIDATES = [
     # wider = the whole workgroup on one sequence
     ([128, 256, BIG], [64, 64, 64]),
     ([BIG], [64]),
+    ([256, BIG], [64, 256]),
     ([384, BIG], [64, 256]),
     ([512, BIG], [64, 256]),
+    ([256, 512, BIG], [64, 128, 256]),
+    ([192, 384, BIG], [64, 128, 256]),
     ([64, 128, 256, BIG], [64, 128, 256, 256]),
     ([128, BIG], [128, 256]),
     ([BIG], [128]),

@@ -95,6 +95,7 @@ int vit_lds(int Lmax) {
 // sequences per workgroup, one wave each, sharing the staged parameter
 // tables.  Wider buckets give the whole workgroup to one sequence.
 constexpr int kPack = 4;
+constexpr int kLanes = 4;  // streams a multi-bucket batch forks onto
 
 // Workgroup width of the kernels that have no packed layout (MEA, Viterbi,
 // gradient): a packed bucket (width 64) gives them a whole workgroup per
@@ -104,152 +105,148 @@ int solo_threads(const cf_batch::Bucket& bk) {
     return bk.Lmax <= 96 ? 128 : 256;
 }
 
-int launch_inside(cf_batch* b) {
+using Bucket = cf_batch::Bucket;
+
+// ---- per-bucket launches: every kernel of one length bucket goes to stream `st`
+
+int launch_inside(cf_batch* b, const Bucket& bk, hipStream_t st) {
     cf_ctx* c = b->ctx;
-    for (const auto& bk : b->buckets) {
-        const int nl = bk.end - bk.begin;
-        if (nl <= 0) continue;
-        // The inside sweep reads its single-branch window from the global FC
-        // table (L2-resident): without the LDS ring a workgroup needs ~14 KB of
-        // LDS at 200 nt instead of ~40 KB, which doubles the resident waves
-        // per CU and beats the ring on throughput (54 vs 87 ms for 3000 x 200 nt).
-        const bool pack = bk.threads <= 64;
-        const int G = pack ? kPack : 1;
-        const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, G);
-        if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the inside kernel's LDS plan");
-        if (pack)
-            hipLaunchKernelGGL((cf_inside_kernel<true>), dim3((nl + G - 1) / G), dim3(64 * G), plan.total,
-                               b->stream, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC,
-                               b->FM, b->FM1, b->FM2, b->FCX, bk.Lmax);
-        else
-            hipLaunchKernelGGL((cf_inside_kernel<false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
-                               b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
-                               b->FM1, b->FM2, b->FCX, bk.Lmax);
-        CF_HIP(c, hipGetLastError(), "inside launch");
-    }
+    const int nl = bk.end - bk.begin;
+    if (nl <= 0) return CF_OK;
+    // The inside sweep reads its single-branch window from the global FC
+    // table (L2-resident): without the LDS ring a workgroup needs ~14 KB of
+    // LDS at 200 nt instead of ~40 KB, which doubles the resident waves
+    // per CU and beats the ring on throughput (54 vs 87 ms for 3000 x 200 nt).
+    const bool pack = bk.threads <= 64;
+    const int G = pack ? kPack : 1;
+    const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, G);
+    if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the inside kernel's LDS plan");
+    if (pack)
+        hipLaunchKernelGGL((cf_inside_kernel<true>), dim3((nl + G - 1) / G), dim3(64 * G), plan.total, st, b->d_seqs,
+                           b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
+                           b->FCX, bk.Lmax);
+    else
+        hipLaunchKernelGGL((cf_inside_kernel<false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
+                           b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
+                           b->FCX, bk.Lmax);
+    CF_HIP(c, hipGetLastError(), "inside launch");
     return CF_OK;
 }
 
-int launch_exterior(cf_batch* b, int mode) {
+int launch_exterior(cf_batch* b, const Bucket& bk, int mode, hipStream_t st) {
     cf_ctx* c = b->ctx;
+    const int nl = bk.end - bk.begin;
+    if (nl <= 0) return CF_OK;
     // one wave per (sequence, chain); up to four waves per workgroup as LDS allows
-    const int per = cf_ext_wave_bytes(b->Lmax);
+    const int per = cf_ext_wave_bytes(bk.Lmax);
     const int avail = c->lds_max - 2048;  // static tables
     if (per > avail) return fail(c, CF_EINVAL, "sequence too long for the exterior kernel");
     const int nw = std::max(1, std::min(4, avail / per));
-    const int tasks = b->n * (mode == 3 ? 2 : 1);
-    hipLaunchKernelGGL(cf_exterior_kernel, dim3((tasks + nw - 1) / nw), dim3(64 * nw), nw * per, b->stream, b->d_seqs,
-                       b->d_order, b->n, b->d_codes, b->d_masks, c->dP32, b->FCX, b->F5, b->F5o, b->logz, mode,
-                       b->Lmax);
+    const int tasks = nl * (mode == 3 ? 2 : 1);
+    hipLaunchKernelGGL(cf_exterior_kernel, dim3((tasks + nw - 1) / nw), dim3(64 * nw), nw * per, st, b->d_seqs,
+                       b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FCX, b->F5, b->F5o, b->logz,
+                       mode, bk.Lmax);
     CF_HIP(c, hipGetLastError(), "exterior launch");
     return CF_OK;
 }
 
-int launch_outside(cf_batch* b) {
+int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st) {
     cf_ctx* c = b->ctx;
-    for (const auto& bk : b->buckets) {
-        const int nl = bk.end - bk.begin;
-        if (nl <= 0) continue;
-        if (bk.threads <= 64) {
-            // packed: no room for a per-sequence FCo ring, the window is read from L2
-            const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, kPack);
-            if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
-            hipLaunchKernelGGL((cf_outside_kernel<false, true>), dim3((nl + kPack - 1) / kPack), dim3(64 * kPack),
-                               plan.total, b->stream, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks,
-                               c->dP32, b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o,
-                               bk.Lmax);
-            CF_HIP(c, hipGetLastError(), "outside launch");
-            continue;
-        }
-        CfLds plan = cf_lds_plan(bk.Lmax, true, 4);
-        const bool ring = plan.total <= c->lds_max;
-        if (!ring) plan = cf_lds_plan(bk.Lmax, false, 4);
-        if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
-        if (ring)
-            hipLaunchKernelGGL((cf_outside_kernel<true, false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
-                               b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
-                               b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
-        else
-            hipLaunchKernelGGL((cf_outside_kernel<false, false>), dim3(nl), dim3(bk.threads), plan.total, b->stream,
-                               b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
-                               b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
-        CF_HIP(c, hipGetLastError(), "outside launch");
-    }
-    return CF_OK;
-}
-
-int launch_post(cf_batch* b) {
-    cf_ctx* c = b->ctx;
-    for (const auto& bk : b->buckets) {
-        const int nl = bk.end - bk.begin;
-        if (nl <= 0) continue;
-        // The kPack waves of a workgroup split one sequence's diagonals: the
-        // kernel waits on global FCo reads, and four times the waves of the
-        // packed form hide that better (12 vs 22 ms for 3000 x 200 nt).
-        const bool pack = false;
+    const int nl = bk.end - bk.begin;
+    if (nl <= 0) return CF_OK;
+    if (bk.threads <= 64) {
+        // packed: no room for a per-sequence FCo ring, the window is read from L2
         const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, kPack);
-        if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the posterior kernel's LDS plan");
-        hipLaunchKernelGGL(cf_post_kernel, dim3(pack ? (nl + kPack - 1) / kPack : nl), dim3(64 * kPack), plan.total,
-                           b->stream, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC,
-                           b->F5, b->F5o, b->FCo, b->FM1o, b->POST, bk.Lmax, pack ? 1 : kPack);
-        CF_HIP(c, hipGetLastError(), "posterior launch");
+        if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
+        hipLaunchKernelGGL((cf_outside_kernel<false, true>), dim3((nl + kPack - 1) / kPack), dim3(64 * kPack),
+                           plan.total, st, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32,
+                           b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
+        CF_HIP(c, hipGetLastError(), "outside launch");
+        return CF_OK;
     }
+    CfLds plan = cf_lds_plan(bk.Lmax, true, 4);
+    const bool ring = plan.total <= c->lds_max;
+    if (!ring) plan = cf_lds_plan(bk.Lmax, false, 4);
+    if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
+    if (ring)
+        hipLaunchKernelGGL((cf_outside_kernel<true, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
+                           b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
+                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
+    else
+        hipLaunchKernelGGL((cf_outside_kernel<false, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
+                           b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
+                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
+    CF_HIP(c, hipGetLastError(), "outside launch");
     return CF_OK;
 }
 
-int launch_mea(cf_batch* b) {
+int launch_post(cf_batch* b, const Bucket& bk, hipStream_t st) {
     cf_ctx* c = b->ctx;
-    for (const auto& bk : b->buckets) {
-        const int nl = bk.end - bk.begin;
-        if (nl <= 0) continue;
-        const int lds = mea_lds(bk.Lmax);
-        if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the MEA kernel");
-        hipLaunchKernelGGL(cf_mea_kernel, dim3(nl * b->n_gamma), dim3(solo_threads(bk)), lds, b->stream, b->d_seqs,
-                           b->d_order + bk.begin, nl, b->d_codes, b->d_masks, b->POST, b->d_gin, b->SC, b->TB,
-                           b->total_T, b->mea_out, b->d_charoff, b->total_chars, bk.Lmax);
-        CF_HIP(c, hipGetLastError(), "mea launch");
-    }
+    const int nl = bk.end - bk.begin;
+    if (nl <= 0) return CF_OK;
+    // The kPack waves of a workgroup split one sequence's diagonals: the
+    // kernel waits on global FCo reads, and four times the waves of the
+    // packed form hide that better (12 vs 22 ms for 3000 x 200 nt).
+    const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, kPack);
+    if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the posterior kernel's LDS plan");
+    hipLaunchKernelGGL(cf_post_kernel, dim3(nl), dim3(64 * kPack), plan.total, st, b->d_seqs, b->d_order + bk.begin,
+                       nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->F5, b->F5o, b->FCo, b->FM1o, b->POST, bk.Lmax,
+                       kPack);
+    CF_HIP(c, hipGetLastError(), "posterior launch");
     return CF_OK;
 }
 
-int launch_viterbi(cf_batch* b, bool lse) {
+int launch_mea(cf_batch* b, const Bucket& bk, hipStream_t st) {
     cf_ctx* c = b->ctx;
-    for (const auto& bk : b->buckets) {
-        const int nl = bk.end - bk.begin;
-        if (nl <= 0) continue;
-        const int lds = vit_lds(bk.Lmax);
-        if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the Viterbi kernel");
-        if (lse)
-            hipLaunchKernelGGL(cf_viterbi_kernel<true>, dim3(nl), dim3(solo_threads(bk)), lds, b->stream, b->d_seqs,
-                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP64, b->VFC, b->VFM, b->VFM1,
-                               b->logz64, b->mfe_out, b->d_charoff, bk.Lmax);
-        else
-            hipLaunchKernelGGL(cf_viterbi_kernel<false>, dim3(nl), dim3(solo_threads(bk)), lds, b->stream, b->d_seqs,
-                               b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP64, b->VFC, b->VFM, b->VFM1,
-                               b->vscore, b->mfe_out, b->d_charoff, bk.Lmax);
-        CF_HIP(c, hipGetLastError(), "viterbi launch");
-    }
+    const int nl = bk.end - bk.begin;
+    if (nl <= 0) return CF_OK;
+    const int lds = mea_lds(bk.Lmax);
+    if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the MEA kernel");
+    hipLaunchKernelGGL(cf_mea_kernel, dim3(nl * b->n_gamma), dim3(solo_threads(bk)), lds, st, b->d_seqs,
+                       b->d_order + bk.begin, nl, b->d_codes, b->d_masks, b->POST, b->d_gin, b->SC, b->TB, b->total_T,
+                       b->mea_out, b->d_charoff, b->total_chars, bk.Lmax);
+    CF_HIP(c, hipGetLastError(), "mea launch");
     return CF_OK;
 }
 
-int launch_sample(cf_batch* b, uint64_t seed) {
+int launch_viterbi(cf_batch* b, const Bucket& bk, bool lse, hipStream_t st) {
     cf_ctx* c = b->ctx;
-    hipLaunchKernelGGL(cf_sample_kernel, dim3(b->sgrid), dim3(256), 0, b->stream, b->d_seqs, b->d_order, b->n,
-                       b->n_samples, seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
-                       b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff);
+    const int nl = bk.end - bk.begin;
+    if (nl <= 0) return CF_OK;
+    const int lds = vit_lds(bk.Lmax);
+    if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the Viterbi kernel");
+    if (lse)
+        hipLaunchKernelGGL(cf_viterbi_kernel<true>, dim3(nl), dim3(solo_threads(bk)), lds, st, b->d_seqs,
+                           b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP64, b->VFC, b->VFM, b->VFM1, b->logz64,
+                           b->mfe_out, b->d_charoff, bk.Lmax);
+    else
+        hipLaunchKernelGGL(cf_viterbi_kernel<false>, dim3(nl), dim3(solo_threads(bk)), lds, st, b->d_seqs,
+                           b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP64, b->VFC, b->VFM, b->VFM1, b->vscore,
+                           b->mfe_out, b->d_charoff, bk.Lmax);
+    CF_HIP(c, hipGetLastError(), "viterbi launch");
+    return CF_OK;
+}
+
+// ---- whole-batch launches
+
+int launch_sample(cf_batch* b, uint64_t seed, hipStream_t st) {
+    cf_ctx* c = b->ctx;
+    hipLaunchKernelGGL(cf_sample_kernel, dim3(b->sgrid), dim3(256), 0, st, b->d_seqs, b->d_order, b->n, b->n_samples,
+                       seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2, b->F5, b->stack,
+                       b->stack_cap, b->samp_out, b->d_soff);
     CF_HIP(c, hipGetLastError(), "sample launch");
     return CF_OK;
 }
 
-int launch_grad(cf_batch* b) {
+int launch_grad(cf_batch* b, hipStream_t st) {
     cf_ctx* c = b->ctx;
-    hipLaunchKernelGGL(cf_fill_f64, dim3(4), dim3(256), 0, b->stream, b->grad, (int64_t)CF_NTAB, 0.0);
+    hipLaunchKernelGGL(cf_fill_f64, dim3(4), dim3(256), 0, st, b->grad, (int64_t)CF_NTAB, 0.0);
     for (const auto& bk : b->buckets) {
         const int nl = bk.end - bk.begin;
         if (nl <= 0) continue;
-        hipLaunchKernelGGL(cf_grad_kernel, dim3(nl), dim3(solo_threads(bk)), 0, b->stream, b->d_seqs,
-                           b->d_order + bk.begin, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
-                           b->F5, b->FCo, b->FMo, b->FM1o, b->F5o, (b->d_gin + kMaxGamma), b->grad);
+        hipLaunchKernelGGL(cf_grad_kernel, dim3(nl), dim3(solo_threads(bk)), 0, st, b->d_seqs, b->d_order + bk.begin,
+                           b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2, b->F5, b->FCo, b->FMo,
+                           b->FM1o, b->F5o, (b->d_gin + kMaxGamma), b->grad);
         CF_HIP(c, hipGetLastError(), "grad launch");
     }
     return CF_OK;
@@ -341,50 +338,91 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
 
 // Enqueue the stages of one run on the context stream (no host sync, no
 // memset/memcpy: the sequence is capturable as a hipGraph).
-int enqueue_run(cf_batch* b, unsigned need, uint64_t seed, bool timing) {
+// The per-bucket stages (inside, exterior, outside, posterior, MEA, Viterbi)
+// of one length bucket, in order, on stream st.
+int enqueue_bucket(cf_batch* b, const Bucket& bk, unsigned need, hipStream_t st) {
+    int rc;
+    if (need & CF_LOGZ) {
+        if ((rc = launch_inside(b, bk, st))) return rc;
+        if ((rc = launch_exterior(b, bk, 1 | ((need & CF_BPP) ? 2 : 0), st))) return rc;
+    } else if (need & CF_BPP) {
+        // inside tables are resident; the outside still needs its exterior chain F5o
+        if ((rc = launch_exterior(b, bk, 2, st))) return rc;
+    }
+    if (need & CF_BPP) {
+        if ((rc = launch_outside(b, bk, st))) return rc;
+        if ((rc = launch_post(b, bk, st))) return rc;
+    }
+    if ((need & CF_MEA) && (rc = launch_mea(b, bk, st))) return rc;
+    if ((need & CF_MFE) && (rc = launch_viterbi(b, bk, false, st))) return rc;
+    if ((need & CF_LOGZ64) && (rc = launch_viterbi(b, bk, true, st))) return rc;
+    return CF_OK;
+}
+
+// Enqueue a run on the batch stream.  With `timing`, stages run one after the
+// other across all buckets and HIP events mark each stage.  Otherwise, when
+// the batch may fork and has several length buckets, each bucket's pipeline
+// runs on its own stream: a bucket of long folds (few workgroups, long serial
+// chains) then overlaps with the short buckets instead of running alone.
+int enqueue_run(cf_batch* b, unsigned need, uint64_t seed, bool timing, bool fork_ok) {
     cf_ctx* c = b->ctx;
     int rc;
-    auto mark = [&](int k) {
-        if (timing) (void)hipEventRecord(b->ev[k], b->stream);
-    };
-    mark(0);
-    if (need & CF_LOGZ) {
-        if ((rc = launch_inside(b))) return rc;
-        mark(1);
-        if ((rc = launch_exterior(b, 1 | ((need & CF_BPP) ? 2 : 0)))) return rc;
+    const int nb = (int)b->buckets.size();
+    if (!timing && fork_ok && b->allow_fork && nb > 1) {
+        const int nlanes = std::min(nb, kLanes);
+        for (int l = (int)b->lanes.size(); l < nlanes; ++l) {
+            hipStream_t s = nullptr;
+            hipEvent_t e = nullptr;
+            CF_HIP(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "lane stream");
+            b->lanes.push_back(s);
+            CF_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming), "lane event");
+            b->lane_ev.push_back(e);
+        }
+        if (!b->fork_ev) CF_HIP(c, hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "fork event");
+        CF_HIP(c, hipEventRecord(b->fork_ev, b->stream), "fork");
+        for (int l = 0; l < nlanes; ++l) CF_HIP(c, hipStreamWaitEvent(b->lanes[l], b->fork_ev, 0), "fork wait");
+        // buckets are longest first: the long ones start first, one per lane
+        for (int k = 0; k < nb; ++k)
+            if ((rc = enqueue_bucket(b, b->buckets[k], need, b->lanes[k % nlanes]))) return rc;
+        for (int l = 0; l < nlanes; ++l) {
+            CF_HIP(c, hipEventRecord(b->lane_ev[l], b->lanes[l]), "join");
+            CF_HIP(c, hipStreamWaitEvent(b->stream, b->lane_ev[l], 0), "join wait");
+        }
     } else {
+        auto mark = [&](int k) {
+            if (timing) (void)hipEventRecord(b->ev[k], b->stream);
+        };
+        mark(0);
+        for (const auto& bk : b->buckets)
+            if ((need & CF_LOGZ) && (rc = launch_inside(b, bk, b->stream))) return rc;
         mark(1);
-        // inside tables are resident; the outside still needs its exterior chain F5o
-        if ((need & CF_BPP) && (rc = launch_exterior(b, 2))) return rc;
+        for (const auto& bk : b->buckets) {
+            if (need & CF_LOGZ) {
+                if ((rc = launch_exterior(b, bk, 1 | ((need & CF_BPP) ? 2 : 0), b->stream))) return rc;
+            } else if ((need & CF_BPP) && (rc = launch_exterior(b, bk, 2, b->stream))) {
+                return rc;
+            }
+        }
+        mark(2);
+        for (const auto& bk : b->buckets)
+            if ((need & CF_BPP) && (rc = launch_outside(b, bk, b->stream))) return rc;
+        mark(3);
+        for (const auto& bk : b->buckets)
+            if ((need & CF_BPP) && (rc = launch_post(b, bk, b->stream))) return rc;
+        mark(4);
+        for (const auto& bk : b->buckets)
+            if ((need & CF_MEA) && (rc = launch_mea(b, bk, b->stream))) return rc;
+        mark(5);
+        for (const auto& bk : b->buckets) {
+            if ((need & CF_MFE) && (rc = launch_viterbi(b, bk, false, b->stream))) return rc;
+            if ((need & CF_LOGZ64) && (rc = launch_viterbi(b, bk, true, b->stream))) return rc;
+        }
     }
-    mark(2);
-    if (need & CF_BPP) {
-        if ((rc = launch_outside(b))) return rc;
-    }
-    mark(3);
-    if (need & CF_BPP) {
-        if ((rc = launch_post(b))) return rc;
-    }
-    mark(4);
-    if (need & CF_MEA) {
-        if ((rc = launch_mea(b))) return rc;
-    }
-    mark(5);
-    if (need & CF_MFE) {
-        if ((rc = launch_viterbi(b, false))) return rc;
-    }
-    if (need & CF_LOGZ64) {
-        if ((rc = launch_viterbi(b, true))) return rc;
-    }
-    mark(6);
-    if (need & CF_SAMPLE) {
-        if ((rc = launch_sample(b, seed))) return rc;
-    }
-    mark(7);
-    if (need & CF_GRAD) {
-        if ((rc = launch_grad(b))) return rc;
-    }
-    mark(8);
+    if (timing) (void)hipEventRecord(b->ev[6], b->stream);
+    if ((need & CF_SAMPLE) && (rc = launch_sample(b, seed, b->stream))) return rc;
+    if (timing) (void)hipEventRecord(b->ev[7], b->stream);
+    if ((need & CF_GRAD) && (rc = launch_grad(b, b->stream))) return rc;
+    if (timing) (void)hipEventRecord(b->ev[8], b->stream);
     return CF_OK;
 }
 
@@ -602,7 +640,7 @@ int batch_enqueue(cf_batch* b, unsigned flags, const float* gammas, int n_gamma,
             if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
             if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
             CF_HIP(c, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-            rc = enqueue_run(b, run, seed, false);
+            rc = enqueue_run(b, run, seed, false, false);  // one stream under capture
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -616,7 +654,7 @@ int batch_enqueue(cf_batch* b, unsigned flags, const float* gammas, int n_gamma,
         }
         CF_HIP(c, hipGraphLaunch(b->gexec, b->stream), "graph launch");
     } else {
-        rc = enqueue_run(b, run, seed, timing);
+        rc = enqueue_run(b, run, seed, timing, true);
         if (rc) return rc;
     }
     CF_HIP(c, hipEventRecord(b->ev[kNumKernels + 2], b->stream), "event");
@@ -736,7 +774,9 @@ int64_t cf_fold_bytes(int L, unsigned flags, int n_gamma, int n_samples) {
 int cf_batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
                     const uint64_t* keys, cf_batch** out) {
     if (!c) return CF_EINVAL;
-    return cfi::batch_create(c, n, seqs, offsets, masks, keys, c->stream, &c->pinned, &c->pinned_bytes, true, out);
+    const int rc = cfi::batch_create(c, n, seqs, offsets, masks, keys, c->stream, &c->pinned, &c->pinned_bytes, true, out);
+    if (rc == CF_OK) (*out)->allow_fork = true;
+    return rc;
 }
 
 int cf_batch_run(cf_batch* b, unsigned flags, const float* gammas, int n_gamma, int n_samples, uint64_t seed,
@@ -885,6 +925,11 @@ void cf_batch_destroy(cf_batch* b) {
     cf_ctx* c = b->ctx;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(b->stream);
+    for (auto s : b->lanes)
+        if (s) (void)hipStreamDestroy(s);
+    for (auto e : b->lane_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (auto& ev : b->ev)

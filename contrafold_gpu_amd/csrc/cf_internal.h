@@ -125,6 +125,11 @@ struct cf_batch {
     hipEvent_t ev[cfi::kNumKernels + 3] = {};  // stage marks 0..8, run start, run end
     float last_ms = 0.0f;
     float kms[cfi::kNumKernels] = {};
+    // per-bucket streams of an eager multi-bucket run (see enqueue_run)
+    bool allow_fork = false;
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> lane_ev;
+    hipEvent_t fork_ev = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     unsigned g_flags = 0;

@@ -818,38 +818,65 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             // Sources in the reference's order: outer rows i' = a-1-l1 ascending
             // (l1 descending), partners descending; in row l1 = 0 the FM1o source
             // of iteration (a-1, b+1) comes after every k >= 1 and before k = 0.
-            for (int l1 = lmax; l1 >= 0; --l1) {
-                const int ip = a - 1 - l1;
+            // The stream of window sources is consumed 8 at a time (their reads
+            // do not depend on acc); exhausted slots fold NEG, which leaves acc
+            // unchanged.
+            if (lmax >= 0) {
+                int l1 = lmax;
+                int ip = a - 1 - l1;
                 uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
-                const int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
-                const uint32_t last = (l1 == 0) ? (bits & 1u) : 0u;
-                bits &= (l1 == 0) ? ~1u : ~0u;
-                auto term = [&](int l2) -> float {
-                    const int dd = d + l1 + l2 + 2;
-                    const float fco = RING ? ring[(dd % CF_RING) * RS + ip] : fco_t[S.dtab[dd] + ip];
-                    float e = S.cs[l1 * 31 + l2] + bpab;
-                    e = e + S.JB[rowJ + S.qo[qbase + l2 - 1]];
+                int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
+                uint32_t last = 0u;
+                if (l1 == 0) {
+                    last = bits & 1u;
+                    bits &= ~1u;
+                }
+                auto term = [&](int r1, int ipr, int rJ, int l2) -> float {
+                    const int dd = d + r1 + l2 + 2;
+                    const float fco = RING ? ring[(dd % CF_RING) * RS + ipr] : fco_t[S.dtab[dd] + ipr];
+                    float e = S.cs[r1 * 31 + l2] + bpab;
+                    e = e + S.JB[rJ + S.qo[qbase + l2 - 1]];
                     e = e + jB2;
-                    const float sn = (l1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
-                                   : (l1 == 1) ? (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f)) : 0.0f;
+                    const float sn = (r1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
+                                   : (r1 == 1) ? (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f)) : 0.0f;
                     e = e + sn;
-                    e = (l1 + l2 == 0) ? e_stack : e;
+                    e = (r1 + l2 == 0) ? e_stack : e;
                     return (fco > CF_HALF) ? fco + e : CF_NEG;
                 };
-                while (bits != 0u) {
-                    const int kk = 31 - __clz(bits);
-                    bits &= ~(1u << kk);
-                    acc = cf_lse(acc, term(kk), C);
-                }
-                if (l1 == 0) {
-                    if (fm1o_src > CF_HALF) {
-                        float v = fm1o_src + jAx;
-                        v = v + mp;
-                        v = v + bpab;
-                        acc = cf_lse(acc, v, C);
+                bool more = true;
+                while (more) {
+                    float v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        while (bits == 0u && l1 > 0) {
+                            --l1;
+                            ip = a - 1 - l1;
+                            bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
+                            rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
+                            if (l1 == 0) {
+                                last = bits & 1u;
+                                bits &= ~1u;
+                            }
+                        }
+                        float val = CF_NEG;
+                        if (bits != 0u) {
+                            const int kk = 31 - __clz(bits);
+                            bits &= ~(1u << kk);
+                            val = term(l1, ip, rowJ, kk);
+                        }
+                        v[u] = val;
                     }
-                    if (last) acc = cf_lse(acc, term(0), C);
+                    more = (bits != 0u) || (l1 > 0);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc = cf_lse(acc, v[u], C);
                 }
+                if (fm1o_src > CF_HALF) {
+                    float v = fm1o_src + jAx;
+                    v = v + mp;
+                    v = v + bpab;
+                    acc = cf_lse(acc, v, C);
+                }
+                if (last) acc = cf_lse(acc, term(0, a - 1, rowJ, 0), C);
             }
             fco_t[db + a] = acc;
             if (RING) ring[(d % CF_RING) * RS + a] = acc;
