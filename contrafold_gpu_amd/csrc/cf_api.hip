@@ -359,6 +359,23 @@ int enqueue_bucket(cf_batch* b, const Bucket& bk, unsigned need, hipStream_t st)
     return CF_OK;
 }
 
+// Streams and events for forking a multi-bucket run (created once per batch,
+// never while a stream is being captured).
+int ensure_lanes(cf_batch* b) {
+    cf_ctx* c = b->ctx;
+    const int nlanes = std::min((int)b->buckets.size(), kLanes);
+    for (int l = (int)b->lanes.size(); l < nlanes; ++l) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        CF_HIP(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "lane stream");
+        b->lanes.push_back(s);
+        CF_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming), "lane event");
+        b->lane_ev.push_back(e);
+    }
+    if (!b->fork_ev) CF_HIP(c, hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "fork event");
+    return CF_OK;
+}
+
 // Enqueue a run on the batch stream.  With `timing`, stages run one after the
 // other across all buckets and HIP events mark each stage.  Otherwise, when
 // the batch may fork and has several length buckets, each bucket's pipeline
@@ -370,15 +387,7 @@ int enqueue_run(cf_batch* b, unsigned need, uint64_t seed, bool timing, bool for
     const int nb = (int)b->buckets.size();
     if (!timing && fork_ok && b->allow_fork && nb > 1) {
         const int nlanes = std::min(nb, kLanes);
-        for (int l = (int)b->lanes.size(); l < nlanes; ++l) {
-            hipStream_t s = nullptr;
-            hipEvent_t e = nullptr;
-            CF_HIP(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "lane stream");
-            b->lanes.push_back(s);
-            CF_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming), "lane event");
-            b->lane_ev.push_back(e);
-        }
-        if (!b->fork_ev) CF_HIP(c, hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "fork event");
+        if ((rc = ensure_lanes(b))) return rc;
         CF_HIP(c, hipEventRecord(b->fork_ev, b->stream), "fork");
         for (int l = 0; l < nlanes; ++l) CF_HIP(c, hipStreamWaitEvent(b->lanes[l], b->fork_ev, 0), "fork wait");
         // buckets are longest first: the long ones start first, one per lane
@@ -639,8 +648,10 @@ int batch_enqueue(cf_batch* b, unsigned flags, const float* gammas, int n_gamma,
         if (!reuse) {
             if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
             if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+            // the per-bucket streams join the capture through the fork/join events
+            if (b->allow_fork && b->buckets.size() > 1 && (rc = ensure_lanes(b))) return rc;
             CF_HIP(c, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-            rc = enqueue_run(b, run, seed, false, false);  // one stream under capture
+            rc = enqueue_run(b, run, seed, false, true);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
