@@ -70,6 +70,15 @@ __host__ __device__ inline int cf_tri32(int L, int i, int j) {
     return d * (L + 1) - ((d * (d - 1)) >> 1) + i;
 }
 __host__ __device__ inline int64_t cf_tri_size(int L) { return (int64_t)(L + 1) * (L + 2) / 2; }
+// Load base[idx] through a 32-bit byte offset: with base in scalar registers
+// the address is one shift and a scalar-base load, no 64-bit lane arithmetic.
+// Valid while idx * sizeof(T) < 2^32 (every per-sequence table offset: a
+// triangle of L <= 32000 has < 5.2e8 cells).
+template <typename T, typename I>
+__device__ __forceinline__ T cf_ld(const T* __restrict__ base, I idx) {
+    return *(const T*)((const char*)base + (size_t)((unsigned)idx * (unsigned)sizeof(T)));
+}
+
 // Exterior (column-major) copy of FC used by the F5 / F5o chains: entry
 // (k, j), 0 <= k < j <= L, holds FC[k+1][j-1].
 __host__ __device__ inline int64_t cf_xbase(int j) { return (int64_t)j * (j - 1) / 2; }

@@ -281,8 +281,8 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
                     float x[8], y[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        x[u] = fm1_t[ia];
-                        y[u] = fm_t[ib];
+                        x[u] = cf_ld(fm1_t, ia);
+                        y[u] = cf_ld(fm_t, ib);
                         ia += L + 1 - (t + u);
                         ib -= L + 1 - (d - (t + u) - 1);
                         ib += 1;
@@ -292,8 +292,8 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
                         fm2 = cf_lse(fm2, (x[u] > CF_HALF && y[u] > CF_HALF) ? x[u] + y[u] : CF_NEG, C);
                 }
                 for (; t <= d - 2; ++t) {
-                    const float x = fm1_t[ia];
-                    const float y = fm_t[ib];
+                    const float x = cf_ld(fm1_t, ia);
+                    const float y = cf_ld(fm_t, ib);
                     fm2 = cf_lse(fm2, (x > CF_HALF && y > CF_HALF) ? x + y : CF_NEG, C);
                     ia += L + 1 - t;
                     ib -= L + 1 - (d - t - 1);
@@ -325,16 +325,41 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
             const float e_stack = S.bp[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
             const float sn01 = T[CF_O_B01 + sj], sn10 = T[CF_O_B01 + si1], sn11 = T[CF_O_I11 + cf_i2(si1, sj)];
             const int pend = min(i + CF_CMAX, j - 2);
-            int p = i, qlo = 0, rowJ = 0;
+            // Rows p = i, i+1 hold the stacking pair, the 0x1 / 1x0 bulges and
+            // the 1x1 interior loop (the only candidates with extra terms);
+            // they are walked one candidate at a time.
+            for (int p = i; p <= min(i + 1, pend); ++p) {
+                const int qlo = max(p + 2, j - CF_CMAX + (p - i));
+                const int rowJ = S.s8[p + 1] * 25 + S.s8[p];
+                uint32_t bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
+                while (bits != 0u) {
+                    const int k = 31 - __clz(bits);
+                    bits &= ~(1u << k);
+                    const int q = qlo + k;
+                    const int l1 = p - i, l2 = j - q;
+                    const float fc = cf_ld(fc_t, S.dtab[d - l1 - l2 - 2] + p + 1);
+                    const float2 bj = S.BJ[(S.qinfo[q] & 0xffff) + rowJ];
+                    float e = S.cs[l1 * 31 + l2] + bj.y;
+                    e = e + jB;
+                    e = e + bj.x;
+                    const float sn = (l1 == 0 && l2 == 1) ? sn01
+                                   : (l1 == 1 && l2 == 0) ? sn10
+                                   : (l1 == 1 && l2 == 1) ? sn11 : 0.0f;
+                    e = e + sn;
+                    if (l1 + l2 == 0) e = e_stack;
+                    acc = cf_lse(acc, (fc > CF_HALF) ? e + fc : CF_NEG, C);
+                }
+            }
+            // Rows p >= i+2: candidates (p ascending, q descending) gathered 8
+            // at a time -- their reads do not depend on acc -- then folded in
+            // order; exhausted slots hold NEG, which leaves acc unchanged.
+            int p = i + 2, qlo = 0, rowJ = 0;
             uint32_t bits = 0u;
             if (p <= pend) {
-                qlo = max(p + 2, j - CF_CMAX);
+                qlo = max(p + 2, j - CF_CMAX + 2);
                 rowJ = S.s8[p + 1] * 25 + S.s8[p];
                 bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
             }
-            // Candidates (p ascending, q descending) are gathered 8 at a time
-            // -- their reads do not depend on acc -- then folded in order;
-            // exhausted slots hold NEG, which leaves acc unchanged.
             bool more = (p <= pend);
             while (more) {
                 float v[8];
@@ -352,18 +377,12 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
                         bits &= ~(1u << k);
                         const int q = qlo + k;
                         const int l1 = p - i, l2 = j - q;
-                        const int dd = d - l1 - l2 - 2;
-                        const float fc = fc_t[S.dtab[dd] + p + 1];
-                        const int qi = S.qinfo[q];
-                        const float2 bj = S.BJ[(qi & 0xffff) + rowJ];
+                        const float fc = cf_ld(fc_t, S.dtab[d - l1 - l2 - 2] + p + 1);
+                        const float2 bj = S.BJ[(S.qinfo[q] & 0xffff) + rowJ];
                         float e = S.cs[l1 * 31 + l2] + bj.y;
                         e = e + jB;
                         e = e + bj.x;
-                        const float sn = (l1 == 0 && l2 == 1) ? sn01
-                                       : (l1 == 1 && l2 == 0) ? sn10
-                                       : (l1 == 1 && l2 == 1) ? sn11 : 0.0f;
-                        e = e + sn;
-                        if (l1 + l2 == 0) e = e_stack;
+                        e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
                         val = (fc > CF_HALF) ? e + fc : CF_NEG;
                     }
                     v[u] = val;
@@ -688,8 +707,8 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
                 for (int w = 0; w < 8; ++w) {
                     const int vv = v - w;
                     g[w] = dom & (vv <= L - 1 - b);
-                    x[w] = fm2o_t[ox + (g[w] ? a : 0)];
-                    y[w] = fm_t[oy + (g[w] ? b : 0)];
+                    x[w] = cf_ld(fm2o_t, ox + (g[w] ? a : 0));
+                    y[w] = cf_ld(fm_t, oy + (g[w] ? b : 0));
                     ox -= L + 2 - (d + vv);
                     oy -= L + 2 - vv;
                 }
@@ -701,8 +720,8 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             }
             for (; v >= 2; --v) {
                 const bool g = dom & (v <= L - 1 - b);
-                const float x = fm2o_t[ox + (g ? a : 0)];
-                const float y = fm_t[oy + (g ? b : 0)];
+                const float x = cf_ld(fm2o_t, ox + (g ? a : 0));
+                const float y = cf_ld(fm_t, oy + (g ? b : 0));
                 ox -= L + 2 - (d + v);
                 oy -= L + 2 - v;
                 const bool ok = g & (x > CF_HALF) & (y > CF_HALF);
@@ -748,8 +767,8 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
                     const int uu = u - w;
                     g[w] = dom & (uu <= a - 1);
                     const int off = g[w] ? a - uu : 0;
-                    x[w] = fm2o_t[(hi ? ox1 : ox0) + off];
-                    y[w] = fm1_t[oy + off];
+                    x[w] = cf_ld(fm2o_t, (hi ? ox1 : ox0) + off);
+                    y[w] = cf_ld(fm1_t, oy + off);
                     ox0 -= L + 2 - (s + uu);
                     ox1 -= L + 2 - (s - 1 + uu);
                     oy -= L + 2 - uu;
@@ -763,8 +782,8 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             for (; u >= 2; --u) {
                 const bool g = dom & (u <= a - 1);
                 const int off = g ? a - u : 0;
-                const float x = fm2o_t[(hi ? ox1 : ox0) + off];
-                const float y = fm1_t[oy + off];
+                const float x = cf_ld(fm2o_t, (hi ? ox1 : ox0) + off);
+                const float y = cf_ld(fm1_t, oy + off);
                 ox0 -= L + 2 - (s + u);
                 ox1 -= L + 2 - (s - 1 + u);
                 oy -= L + 2 - u;
@@ -818,65 +837,77 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
             // Sources in the reference's order: outer rows i' = a-1-l1 ascending
             // (l1 descending), partners descending; in row l1 = 0 the FM1o source
             // of iteration (a-1, b+1) comes after every k >= 1 and before k = 0.
-            // The stream of window sources is consumed 8 at a time (their reads
-            // do not depend on acc); exhausted slots fold NEG, which leaves acc
-            // unchanged.
-            if (lmax >= 0) {
+            // Rows l1 >= 2 (no stacking, bulge or 1x1 extras) stream 8 sources at
+            // a time -- their reads do not depend on acc; exhausted slots fold
+            // NEG, which leaves acc unchanged -- then rows 1 and 0 follow one
+            // source at a time.
+            auto term = [&](int r1, int ipr, int rJ, int l2) -> float {
+                const int dd = d + r1 + l2 + 2;
+                const float fco = RING ? ring[(dd % CF_RING) * RS + ipr] : cf_ld(fco_t, S.dtab[dd] + ipr);
+                float e = S.cs[r1 * 31 + l2] + bpab;
+                e = e + S.JB[rJ + S.qo[qbase + l2 - 1]];
+                e = e + jB2;
+                const float sn = (r1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
+                               : (r1 == 1) ? (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f)) : 0.0f;
+                e = e + sn;
+                e = (r1 + l2 == 0) ? e_stack : e;
+                return (fco > CF_HALF) ? fco + e : CF_NEG;
+            };
+            if (lmax >= 2) {
                 int l1 = lmax;
                 int ip = a - 1 - l1;
                 uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
                 int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
-                uint32_t last = 0u;
-                if (l1 == 0) {
-                    last = bits & 1u;
-                    bits &= ~1u;
-                }
-                auto term = [&](int r1, int ipr, int rJ, int l2) -> float {
-                    const int dd = d + r1 + l2 + 2;
-                    const float fco = RING ? ring[(dd % CF_RING) * RS + ipr] : fco_t[S.dtab[dd] + ipr];
-                    float e = S.cs[r1 * 31 + l2] + bpab;
-                    e = e + S.JB[rJ + S.qo[qbase + l2 - 1]];
-                    e = e + jB2;
-                    const float sn = (r1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
-                                   : (r1 == 1) ? (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f)) : 0.0f;
-                    e = e + sn;
-                    e = (r1 + l2 == 0) ? e_stack : e;
-                    return (fco > CF_HALF) ? fco + e : CF_NEG;
-                };
                 bool more = true;
                 while (more) {
                     float v[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        while (bits == 0u && l1 > 0) {
+                        while (bits == 0u && l1 > 2) {
                             --l1;
                             ip = a - 1 - l1;
                             bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
                             rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
-                            if (l1 == 0) {
-                                last = bits & 1u;
-                                bits &= ~1u;
-                            }
                         }
                         float val = CF_NEG;
                         if (bits != 0u) {
-                            const int kk = 31 - __clz(bits);
-                            bits &= ~(1u << kk);
-                            val = term(l1, ip, rowJ, kk);
+                            const int l2 = 31 - __clz(bits);
+                            bits &= ~(1u << l2);
+                            const int dd = d + l1 + l2 + 2;
+                            const float fco = RING ? ring[(dd % CF_RING) * RS + ip] : cf_ld(fco_t, S.dtab[dd] + ip);
+                            float e = S.cs[l1 * 31 + l2] + bpab;
+                            e = e + S.JB[rowJ + S.qo[qbase + l2 - 1]];
+                            e = e + jB2;
+                            e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
+                            val = (fco > CF_HALF) ? fco + e : CF_NEG;
                         }
                         v[u] = val;
                     }
-                    more = (bits != 0u) || (l1 > 0);
+                    more = (bits != 0u) || (l1 > 2);
 #pragma unroll
                     for (int u = 0; u < 8; ++u) acc = cf_lse(acc, v[u], C);
                 }
-                if (fm1o_src > CF_HALF) {
-                    float v = fm1o_src + jAx;
-                    v = v + mp;
-                    v = v + bpab;
-                    acc = cf_lse(acc, v, C);
+            }
+            for (int l1 = min(lmax, 1); l1 >= 0; --l1) {
+                const int ip = a - 1 - l1;
+                uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
+                const int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
+                const uint32_t last = (l1 == 0) ? (bits & 1u) : 0u;
+                if (l1 == 0) bits &= ~1u;
+                while (bits != 0u) {
+                    const int kk = 31 - __clz(bits);
+                    bits &= ~(1u << kk);
+                    acc = cf_lse(acc, term(l1, ip, rowJ, kk), C);
                 }
-                if (last) acc = cf_lse(acc, term(0, a - 1, rowJ, 0), C);
+                if (l1 == 0) {
+                    if (fm1o_src > CF_HALF) {
+                        float v = fm1o_src + jAx;
+                        v = v + mp;
+                        v = v + bpab;
+                        acc = cf_lse(acc, v, C);
+                    }
+                    if (last) acc = cf_lse(acc, term(0, ip, rowJ, 0), C);
+                }
             }
             fco_t[db + a] = acc;
             if (RING) ring[(d % CF_RING) * RS + a] = acc;
@@ -962,7 +993,7 @@ __global__ void __launch_bounds__(256) cf_post_kernel(
                 const int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
                 auto term = [&](int l2) -> float {
                     const int d = s + l1 + l2 + 2;
-                    const float outside = fco_t[S.dtab[d] + ip] - Z;
+                    const float outside = cf_ld(fco_t, S.dtab[d] + ip) - Z;
                     float e;
                     if (l1 + l2 == 0) {
                         e = outside + bpab;
@@ -1232,8 +1263,8 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                     double x[8], y[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        x[u] = fm1_t[ia];
-                        y[u] = fm_t[ib];
+                        x[u] = cf_ld(fm1_t, ia);
+                        y[u] = cf_ld(fm_t, ib);
                         ia += L + 1 - (t + u);
                         ib -= L + 1 - (s - (t + u) - 1);
                         ib += 1;
