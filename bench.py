@@ -161,6 +161,37 @@ def run_config(name, args):
     print(json.dumps(out), flush=True)
 
 
+def published_workloads(cf, workloads, P, log, reps=3):
+    """The two workloads the reference publishes GPU figures for, run through
+    the drop-in API end to end (host strings in, results on the host, like the
+    reference's timed calls): logZ of 10,000 x 200 nt uniform ACGU, seed 0
+    (Benchmark.md:14-20, 34: 9,830 folds/s on an RTX 5090) and 100 x 200 nt
+    x 100 Boltzmann samples (Benchmark.md:11, 68: 5,627 samples/s).  Other
+    hardware, so a comparison, never vs_baseline."""
+    seqs = workloads.uniform(10000, 200, seed=0)
+    cf.logZ_batch(seqs, P)  # warm: library, parameters, code objects
+    t = time.perf_counter()
+    for _ in range(reps):
+        cf.logZ_batch(seqs, P)
+    logz_fps = len(seqs) * reps / (time.perf_counter() - t)
+    log(f"[bench] logZ_batch 10000 x 200 nt uniform: {logz_fps:.0f} folds/s end to end")
+    sseqs = seqs[:100]
+    cf.sample_batch(sseqs, P, 100, seed=1)
+    t = time.perf_counter()
+    for k in range(reps):
+        cf.sample_batch(sseqs, P, 100, seed=2 + k)
+    sps = len(sseqs) * 100 * reps / (time.perf_counter() - t)
+    log(f"[bench] sample_batch 100 x 200 nt x 100: {sps:.0f} samples/s end to end")
+    return {
+        "logz_10000x200nt_uniform": {"folds_per_s": round(logz_fps, 1), "reference_rtx5090": 9830,
+                                      "ratio": round(logz_fps / 9830, 2)},
+        "sample_100x200nt_x100": {"samples_per_s": round(sps, 1), "reference_rtx5090": 5627,
+                                  "ratio": round(sps / 5627, 2)},
+        "note": "drop-in API calls timed end to end (encode, upload, fold, download); reference figures are "
+                "the RTX 5090 numbers its Benchmark.md publishes",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
@@ -233,6 +264,17 @@ def main():
         # sampling dominates: report it against HBM with the bytes it must read
         roof = {"kernel": dominant, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None}
+    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE on this workload and calibrated on a copy-like
+    # kernel of known size (profiles/c2_10000x200nt_traffic.json)
+    tpath = os.path.join(ROOT, "profiles", "c2_10000x200nt_traffic.json")
+    if os.path.exists(tpath) and args.n == 10000:
+        with open(tpath) as fh:
+            tdoc = json.load(fh)
+        for name, rec in tdoc.get("kernels", {}).items():
+            if name.startswith(f"cf_{dominant}_kernel"):
+                roof["traffic"] = rec["bytes_per_launch"]
+                roof["traffic_unit"] = "bytes/launch"
     per_kernel = {k: {"ms": round(v, 3),
                       "tflops": (round(flops[k] / (v * 1e-3) / 1e12, 3) if k in flops and v > 0 else None)}
                   for k, v in kavg.items()}
@@ -246,6 +288,7 @@ def main():
     mea_fps = args.n * mea_steps / (time.perf_counter() - t)
     log(f"[bench] MEA folds/s: {mea_fps:.0f}")
     batch.close()
+    published = published_workloads(cf, workloads, P, log)
 
     out = {
         "metric": "folds/s @200 nt (served vs plain batch) + p50/p99 latency, logZ err <=5e-4",
@@ -259,6 +302,7 @@ def main():
         "roofline": roof,
         "kernels": per_kernel,
         "mea": {"folds_per_s": round(mea_fps, 2), "gamma": 6.0},
+        "published_workloads": published,
     }
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, all host cores) ...")

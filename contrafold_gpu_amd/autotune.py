@@ -16,6 +16,9 @@ BIG = 1 << 30
 CANDIDATES = [
     # width 64 = packed (one wave per sequence, 4 sequences per workgroup);
     # wider = the whole workgroup on one sequence
+    ([128, 384, BIG], [64, 64, 512]),
+    ([384, BIG], [64, 1024]),
+    ([256, 512, BIG], [64, 256, 1024]),
     ([128, 256, BIG], [64, 64, 64]),
     ([BIG], [64]),
     ([256, BIG], [64, 256]),
@@ -28,7 +31,7 @@ CANDIDATES = [
     ([BIG], [128]),
     ([BIG], [256]),
 ]
-DEFAULT = CANDIDATES[0]
+DEFAULT = CANDIDATES[0]  # the library's built-in tuning
 
 
 def cache_path():
@@ -65,12 +68,17 @@ def _save_cache(data):
 
 def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, sample=1500, repeats=2,
          candidates=None, use_cache=True, seed=0, log=None):
-    """Pick (edges, threads) for this workload; installs and returns it."""
+    """Pick (edges, threads) for this workload; installs and returns it.
+
+    The sample is at least a third of the workload (up to 8,000 sequences):
+    a small sample leaves the GPU half empty, where wide teams win on latency,
+    while the whole batch fills it and favours narrower teams."""
     from ._lib import CF_BPP, CF_MEA, CF_SAMPLE
     flags = CF_BPP if flags is None else flags
     rng = np.random.default_rng(seed)
     n = len(seqs)
-    pick = rng.choice(n, size=min(sample, n), replace=False)
+    size = min(n, max(sample, min(n // 3, 8000)))
+    pick = rng.choice(n, size=size, replace=False)
     sub = [seqs[k] for k in pick]
     subm = [masks[k] for k in pick] if masks is not None else None
     key = profile_key([len(s) for s in sub], flags)

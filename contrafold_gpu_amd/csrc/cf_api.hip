@@ -101,7 +101,7 @@ constexpr int kLanes = 4;  // streams a multi-bucket batch forks onto
 // gradient): a packed bucket (width 64) gives them a whole workgroup per
 // sequence, sized to the bucket's longest sequence.
 int solo_threads(const cf_batch::Bucket& bk) {
-    if (bk.threads > 64) return bk.threads;
+    if (bk.threads > 64) return std::min(bk.threads, 256);
     return bk.Lmax <= 96 ? 128 : 256;
 }
 
@@ -187,11 +187,15 @@ int launch_post(cf_batch* b, const Bucket& bk, hipStream_t st) {
     // The kPack waves of a workgroup split one sequence's diagonals: the
     // kernel waits on global FCo reads, and four times the waves of the
     // packed form hide that better (12 vs 22 ms for 3000 x 200 nt).
-    const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, kPack);
+    // A wide bucket's workgroup width (up to 16 waves) splits long folds
+    // further, as far as the per-wave LDS areas fit.
+    int split = std::max(kPack, bk.threads / 64);
+    CfLds plan = cf_lds_plan(bk.Lmax, false, 4, split);
+    while (plan.total > c->lds_max && split > kPack) plan = cf_lds_plan(bk.Lmax, false, 4, --split);
     if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the posterior kernel's LDS plan");
-    hipLaunchKernelGGL(cf_post_kernel, dim3(nl), dim3(64 * kPack), plan.total, st, b->d_seqs, b->d_order + bk.begin,
+    hipLaunchKernelGGL(cf_post_kernel, dim3(nl), dim3(64 * split), plan.total, st, b->d_seqs, b->d_order + bk.begin,
                        nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->F5, b->F5o, b->FCo, b->FM1o, b->POST, bk.Lmax,
-                       kPack);
+                       split);
     CF_HIP(c, hipGetLastError(), "posterior launch");
     return CF_OK;
 }
@@ -761,7 +765,7 @@ int cf_set_tuning(cf_ctx* c, const int* edges, const int* threads, int nb) {
     if (!c || !edges || !threads || nb <= 0) return fail(c, CF_EINVAL, "bad tuning arguments");
     std::vector<int> e(edges, edges + nb), t(threads, threads + nb);
     for (int k = 0; k < nb; ++k) {
-        if (t[k] < 64 || t[k] > 256 || t[k] % 64) return fail(c, CF_EINVAL, "threads must be 64, 128, 192 or 256");
+        if (t[k] < 64 || t[k] > 1024 || t[k] % 64) return fail(c, CF_EINVAL, "threads must be a multiple of 64 in 64..1024");
         if (k && e[k] <= e[k - 1]) return fail(c, CF_EINVAL, "bucket edges must increase");
     }
     e.back() = 1 << 30;

@@ -61,8 +61,8 @@ struct cf_ctx {
     CfParams<double>* dP64 = nullptr;
     uint64_t params_version = 0;
     // bucket edges (max length) and team width per bucket; width 64 = packed
-    std::vector<int> edges{128, 256, 1 << 30};
-    std::vector<int> threads{64, 64, 64};
+    std::vector<int> edges{128, 384, 1 << 30};
+    std::vector<int> threads{64, 64, 512};
     int lds_max = 65536;
     int n_cu = 256;
     cfi::DevPool pool;

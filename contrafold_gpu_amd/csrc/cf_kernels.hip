@@ -228,7 +228,7 @@ __device__ __forceinline__ bool cf_allow(const CfShared& S, int a, int b) {
 // Each cell's terms are folded in the reference's order, so every table is
 // bit-identical to a one-diagonal-at-a-time sweep.
 template <bool PACK>
-__global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
+__global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
                                                         int nseq, const uint8_t* __restrict__ codes,
                                                         const uint8_t* __restrict__ masks,
                                                         const CfParams<float>* __restrict__ P, float* __restrict__ FC,
@@ -260,53 +260,73 @@ __global__ void __launch_bounds__(256) cf_inside_kernel(const CfSeq* __restrict_
         const int step = s >> 1;
         const int n0 = L + 1 - s;
         const int n1 = (s + 1 <= L) ? L - s : 0;
-        // ---- A: multiloop split FM2 of diagonals s and s+1; collect pairable cells
-        for (int c = tid; c < n0 + n1; c += NT) {
-            const int hi = (c >= n0) ? 1 : 0;
-            const int d = s + hi;
-            const int i = c - (hi ? n0 : 0);
-            const int j = i + d;
-            const bool inner = (i > 0) && (j < L);
-            const bool alw = inner && cf_allow(S, i, j + 1);
-            const int db = S.dtab[d];
-            float fm2 = CF_NEG;
-            if (inner && d >= 4) {
-                // FM2[i][j] = LSE_{k=i+2..j-2} FM1[i][k] + FM[k][j]  (k = i + t).
-                // Loads of 8 consecutive terms are issued before the 8 folds,
-                // so memory latency overlaps; the fold order is unchanged.
+        // ---- A: multiloop split FM2 of diagonals s and s+1; collect pairable cells.
+        // A lane takes row i of both diagonals: cells (i, i+s) and (i, i+s+1)
+        // fold the same FM1[i][i+t] against FM[i+t][i+s] and FM[i+t][i+s+1],
+        // so the row is read once for two cells (the second column is the
+        // neighbouring lane's first, one step earlier, and hits the cache).
+        for (int i = tid; i < n0; i += NT) {
+            const int j0 = i + s;
+            const bool inner0 = (i > 0) && (j0 < L);
+            const bool inner1 = (i < n1) && (i > 0) && (j0 + 1 < L);  // implies inner0
+            float a0 = CF_NEG, a1 = CF_NEG;
+            if (inner0 && s >= 3) {
+                // FM2[i][j] = LSE_{k=i+2..j-2} FM1[i][k] + FM[k][j]  (k = i + t):
+                // t = 2..s-2 for both cells, then t = s-1 for (i, i+s+1) alone.
+                // Loads of 8 consecutive terms are issued before their folds, so
+                // memory latency overlaps; each cell's fold order is unchanged.
                 int ia = S.dtab[2] + i;
-                int ib = S.dtab[d - 2] + i + 2;
+                int ib0 = S.dtab[s - 2] + i + 2;
+                int ib1 = S.dtab[s - 1] + i + 2;
                 int t = 2;
-                for (; t + 7 <= d - 2; t += 8) {
-                    float x[8], y[8];
+                for (; t + 7 <= s - 2; t += 8) {
+                    float x[8], y0[8], y1[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         x[u] = cf_ld(fm1_t, ia);
-                        y[u] = cf_ld(fm_t, ib);
+                        y0[u] = cf_ld(fm_t, ib0);
+                        y1[u] = cf_ld(fm_t, ib1);
                         ia += L + 1 - (t + u);
-                        ib -= L + 1 - (d - (t + u) - 1);
-                        ib += 1;
+                        ib0 -= L - (s - (t + u) - 1);
+                        ib1 -= L - (s - (t + u));
                     }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        fm2 = cf_lse(fm2, (x[u] > CF_HALF && y[u] > CF_HALF) ? x[u] + y[u] : CF_NEG, C);
+                    for (int u = 0; u < 8; ++u) {
+                        a0 = cf_lse(a0, (x[u] > CF_HALF && y0[u] > CF_HALF) ? x[u] + y0[u] : CF_NEG, C);
+                        a1 = cf_lse(a1, (x[u] > CF_HALF && y1[u] > CF_HALF) ? x[u] + y1[u] : CF_NEG, C);
+                    }
                 }
-                for (; t <= d - 2; ++t) {
+                for (; t <= s - 2; ++t) {
                     const float x = cf_ld(fm1_t, ia);
-                    const float y = cf_ld(fm_t, ib);
-                    fm2 = cf_lse(fm2, (x > CF_HALF && y > CF_HALF) ? x + y : CF_NEG, C);
+                    const float y0 = cf_ld(fm_t, ib0);
+                    const float y1 = cf_ld(fm_t, ib1);
+                    a0 = cf_lse(a0, (x > CF_HALF && y0 > CF_HALF) ? x + y0 : CF_NEG, C);
+                    a1 = cf_lse(a1, (x > CF_HALF && y1 > CF_HALF) ? x + y1 : CF_NEG, C);
                     ia += L + 1 - t;
-                    ib -= L + 1 - (d - t - 1);
-                    ib += 1;
+                    ib0 -= L - (s - t - 1);
+                    ib1 -= L - (s - t);
+                }
+                if (inner1) {
+                    const float x = cf_ld(fm1_t, ia);
+                    const float y1 = cf_ld(fm_t, ib1);
+                    a1 = cf_lse(a1, (x > CF_HALF && y1 > CF_HALF) ? x + y1 : CF_NEG, C);
                 }
             }
-            fm2_t[db + i] = fm2;
-            if (alw) {
-                const int slotn = atomicAdd(&S.cnt[step & 1], 1);
-                S.alist[slotn] = (uint16_t)(i | (hi << 15));
-            } else {
-                fc_t[db + i] = CF_NEG;
-                if (inner) fcx_t[cf_xbase(j + 1) + i - 1] = CF_NEG;
+#pragma unroll
+            for (int hi = 0; hi < 2; ++hi) {
+                if (hi == 1 && i >= n1) break;
+                const int d = s + hi;
+                const int j = j0 + hi;
+                const bool inner = hi ? inner1 : inner0;
+                const int db = S.dtab[d];
+                fm2_t[db + i] = inner ? (hi ? a1 : a0) : CF_NEG;
+                if (inner && cf_allow(S, i, j + 1)) {
+                    const int slotn = atomicAdd(&S.cnt[step & 1], 1);
+                    S.alist[slotn] = (uint16_t)(i | (hi << 15));
+                } else {
+                    fc_t[db + i] = CF_NEG;
+                    if (inner) fcx_t[cf_xbase(j + 1) + i - 1] = CF_NEG;
+                }
             }
         }
         cf_team_sync<PACK>();
@@ -616,7 +636,7 @@ __global__ void __launch_bounds__(256) cf_exterior_kernel(const CfSeq* __restric
 // Each cell's terms are folded in the reference's order, so every table is
 // bit-identical to a one-diagonal-at-a-time sweep.
 template <bool RING, bool PACK>
-__global__ void __launch_bounds__(256) cf_outside_kernel(
+__global__ void __launch_bounds__(1024) cf_outside_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ order, int nseq, const uint8_t* __restrict__ codes,
     const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
     const float* __restrict__ FM, const float* __restrict__ FM1, const float* __restrict__ F5,
@@ -739,61 +759,70 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
         const int step = (L - s) >> 1;
         const int n0 = L + 1 - s;
         const int n1 = (s >= 1) ? L + 2 - s : 0;  // cells of diagonal s-1
-        // ---- A: split part of FMo for diagonals s and s-1; collect pairable cells
-        for (int c0 = __builtin_amdgcn_readfirstlane(tid - lane); c0 < n0 + n1; c0 += NT) {
-            const int c = c0 + lane;
-            const bool live = c < n0 + n1;
-            const int hi = (c >= n0) ? 1 : 0;
-            const int d = s - hi;
-            const int a = c - (hi ? n0 : 0);
-            const int b = a + d;
-            const bool dom = live && (a > 0) && (a + 2 <= b) && (b < L);
-            const bool alw = live && (a > 0) && (b < L) && cf_allow(S, a, b + 1);
-            float fmo = CF_NEG;
+        // ---- A: split part of FMo for diagonals s and s-1; collect pairable cells.
+        // A lane takes column a of both diagonals: cells (a, a+s) and
+        // (a, a+s-1) fold FM2o of their own rows against the same FM1[i'][a],
+        // so that column is read once for two cells.
+        const int na = max(n0, n1);
+        for (int c0 = __builtin_amdgcn_readfirstlane(tid - lane); c0 < na; c0 += NT) {
+            const int a = c0 + lane;
+            const bool live0 = a < n0, live1 = a < n1;
+            const bool dom0 = live0 && (a > 0) && (s >= 2) && (a + s < L);
+            const bool dom1 = live1 && (a > 0) && (s >= 3) && (a + s - 1 < L);
+            float f0 = CF_NEG, f1 = CF_NEG;
             // FM2o(i', b) + FM1[i'][a] for i' = 1 .. a-2 ascending, i.e. u = a - i'
             // descending from the pass's largest a.  Wave-uniform offsets of the
-            // diagonals d + u (FM2o, one sequence per diagonal of the pair) and
-            // u (FM1) step down with u; lanes outside the domain read cell 0
-            // of the diagonal and fold NEG, which leaves the sum unchanged.
-            const int cl = c0 + WAVE - 1;
-            const int amax = max(c0 < n0 ? min(cl, n0 - 1) : -1, cl >= n0 ? min(cl - n0, n1 - 1) : -1);
+            // diagonals s + u, s - 1 + u (FM2o) and u (FM1) step down with u;
+            // lanes outside a domain read cell 0 of the diagonal and fold NEG,
+            // which leaves the sum unchanged.
+            const int amax = min(c0 + WAVE - 1, na - 1);
             int u = amax - 1;
             int ox0 = (int)cf_dbase(L, s + u), ox1 = (int)cf_dbase(L, s - 1 + u), oy = (int)cf_dbase(L, u);
             for (; u - 7 >= 2; u -= 8) {
-                float x[8], y[8];
-                bool g[8];
+                float x0[8], x1[8], y[8];
+                bool g0[8], g1[8];
 #pragma unroll
                 for (int w = 0; w < 8; ++w) {
                     const int uu = u - w;
-                    g[w] = dom & (uu <= a - 1);
-                    const int off = g[w] ? a - uu : 0;
-                    x[w] = cf_ld(fm2o_t, (hi ? ox1 : ox0) + off);
-                    y[w] = cf_ld(fm1_t, oy + off);
+                    g0[w] = dom0 & (uu <= a - 1);
+                    g1[w] = dom1 & (uu <= a - 1);
+                    x0[w] = cf_ld(fm2o_t, ox0 + (g0[w] ? a - uu : 0));
+                    x1[w] = cf_ld(fm2o_t, ox1 + (g1[w] ? a - uu : 0));
+                    y[w] = cf_ld(fm1_t, oy + ((g0[w] | g1[w]) ? a - uu : 0));
                     ox0 -= L + 2 - (s + uu);
                     ox1 -= L + 2 - (s - 1 + uu);
                     oy -= L + 2 - uu;
                 }
 #pragma unroll
                 for (int w = 0; w < 8; ++w) {
-                    const bool ok = g[w] & (x[w] > CF_HALF) & (y[w] > CF_HALF);
-                    fmo = cf_lse(fmo, ok ? x[w] + y[w] : CF_NEG, C);
+                    const bool ok0 = g0[w] & (x0[w] > CF_HALF) & (y[w] > CF_HALF);
+                    const bool ok1 = g1[w] & (x1[w] > CF_HALF) & (y[w] > CF_HALF);
+                    f0 = cf_lse(f0, ok0 ? x0[w] + y[w] : CF_NEG, C);
+                    f1 = cf_lse(f1, ok1 ? x1[w] + y[w] : CF_NEG, C);
                 }
             }
             for (; u >= 2; --u) {
-                const bool g = dom & (u <= a - 1);
-                const int off = g ? a - u : 0;
-                const float x = cf_ld(fm2o_t, (hi ? ox1 : ox0) + off);
-                const float y = cf_ld(fm1_t, oy + off);
+                const bool g0 = dom0 & (u <= a - 1);
+                const bool g1 = dom1 & (u <= a - 1);
+                const float x0 = cf_ld(fm2o_t, ox0 + (g0 ? a - u : 0));
+                const float x1 = cf_ld(fm2o_t, ox1 + (g1 ? a - u : 0));
+                const float y = cf_ld(fm1_t, oy + ((g0 | g1) ? a - u : 0));
                 ox0 -= L + 2 - (s + u);
                 ox1 -= L + 2 - (s - 1 + u);
                 oy -= L + 2 - u;
-                const bool ok = g & (x > CF_HALF) & (y > CF_HALF);
-                fmo = cf_lse(fmo, ok ? x + y : CF_NEG, C);
+                const bool ok0 = g0 & (x0 > CF_HALF) & (y > CF_HALF);
+                const bool ok1 = g1 & (x1 > CF_HALF) & (y > CF_HALF);
+                f0 = cf_lse(f0, ok0 ? x0 + y : CF_NEG, C);
+                f1 = cf_lse(f1, ok1 ? x1 + y : CF_NEG, C);
             }
-            if (live) {
+#pragma unroll
+            for (int hi = 0; hi < 2; ++hi) {
+                if (!(hi ? live1 : live0)) continue;
+                const int d = s - hi;
+                const int b = a + d;
                 const int db = S.dtab[d];
-                fmo_t[db + a] = fmo;
-                if (alw) {
+                fmo_t[db + a] = hi ? f1 : f0;
+                if ((a > 0) && (b < L) && cf_allow(S, a, b + 1)) {
                     const int slotn = atomicAdd(&S.cnt[step & 1], 1);
                     S.alist[slotn] = (uint16_t)(a | (hi << 15));
                 } else {
@@ -937,7 +966,7 @@ __global__ void __launch_bounds__(256) cf_outside_kernel(
 // split > 1 the waves of a workgroup share one sequence and take every
 // split-th diagonal (lower latency for a lone long fold).
 // POSTF is indexed by FC cell: POSTF[tri(a, b)] = P(a pairs b+1).
-__global__ void __launch_bounds__(256) cf_post_kernel(
+__global__ void __launch_bounds__(1024) cf_post_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ order, int nseq, const uint8_t* __restrict__ codes,
     const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
     const float* __restrict__ F5, const float* __restrict__ F5o, const float* __restrict__ FCo,

@@ -32,7 +32,7 @@ struct Req {
     float gamma = 6.0f;
     int n_samples = 0;
     uint64_t seed = 0;
-    double arrival_us = 0.0, done_us = 0.0;
+    double arrival_us = 0.0, launch_us = 0.0, done_us = 0.0;
     int state = 0;  // 0 pending, 1 in flight, 2 done, 3 failed
     int64_t bytes = 0;
     std::vector<char> result;
@@ -52,6 +52,7 @@ struct Slot {
     std::vector<int64_t> reqs;
     unsigned kind = 0;
     int64_t bytes = 0;
+    bool long_class = false;
 };
 
 // Length classes: a batch retires when its slowest fold finishes, so a batch
@@ -128,12 +129,35 @@ int retire(cf_server* s, Slot& sl) {
     return CF_OK;
 }
 
+// Workgroup width for a small batch whose longest fold has length Lmax: each
+// fold gets a whole workgroup sized to its diagonals (latency over packing).
+int serve_width(int Lmax) { return Lmax <= 100 ? 128 : Lmax <= 250 ? 256 : Lmax <= 500 ? 512 : 1024; }
+
+// Folds of length class >= kLongClass take long; at most half of the slots
+// may hold them, so short requests always find a slot.
+constexpr int kLongClass = 3;
+
 int launch(cf_server* s, Slot& sl) {
     cf_ctx* c = s->ctx;
-    // head-of-line request decides the batch kind
     while (!s->pending.empty() && s->reqs[s->pending.front()].state != 0) s->pending.pop_front();
     if (s->pending.empty()) return 0;
-    const Req& head = s->reqs[s->pending.front()];
+    // the oldest request decides the batch kind and length class, unless it is
+    // long and the long folds already hold their share of the slots
+    int long_busy = 0;
+    for (const auto& o : s->slots)
+        if (o.batch && o.long_class) ++long_busy;
+    const bool long_ok = long_busy < (int)s->slots.size() / 2;
+    int64_t head_id = -1;
+    for (int64_t id : s->pending) {
+        const Req& r = s->reqs[id];
+        if (r.state != 0) continue;
+        if (long_ok || length_class((int)r.seq.size()) < kLongClass) {
+            head_id = id;
+            break;
+        }
+    }
+    if (head_id < 0) return 0;
+    const Req& head = s->reqs[head_id];
     const unsigned kind = head.kind;
     const float gamma = head.gamma;
     const int nsamp = head.n_samples;
@@ -143,7 +167,8 @@ int launch(cf_server* s, Slot& sl) {
         r.state = 3;
         r.done_us = s->now_us();
         s->completed.push_back(r.id);
-        s->pending.pop_front();
+        // leave the queue now: cf_server_result erases the request
+        s->pending.erase(std::find(s->pending.begin(), s->pending.end(), head_id));
         return 0;
     }
     std::vector<int64_t> pick;
@@ -184,10 +209,13 @@ int launch(cf_server* s, Slot& sl) {
         if (!r.mask.empty()) std::memcpy(masks.data() + offs[k], r.mask.data(), r.mask.size());
         keys[k] = r.seed;  // the request's own stream key: independent of the batch it joins
         r.state = 1;
+        r.launch_us = s->now_us();
     }
     // A small batch cannot fill the GPU with one wave per fold; giving each
-    // fold a whole 256-thread workgroup cuts its latency instead.
-    const int width = (n <= 4 * c->n_cu) ? 256 : 0;
+    // fold a whole workgroup cuts its latency instead.
+    int lmax = 0;
+    for (int k = 0; k < n; ++k) lmax = std::max(lmax, (int)(offs[k + 1] - offs[k]));
+    const int width = (n <= 4 * c->n_cu) ? serve_width(lmax) : 0;
     cf_batch* b = nullptr;
     int rc = cfi::batch_create(c, n, seqs.data(), offs.data(), any_mask ? masks.data() : nullptr, keys.data(),
                                sl.stream, &sl.stage_in, &sl.stage_in_bytes, false, &b, width);
@@ -245,6 +273,7 @@ int launch(cf_server* s, Slot& sl) {
     sl.batch = b;
     sl.reqs = pick;
     sl.kind = kind;
+    sl.long_class = lclass >= kLongClass;
     sl.bytes = bytes;
     s->in_use += bytes;
     return n;
@@ -343,7 +372,7 @@ int cf_server_pending(const cf_server* s) {
 
 int cf_server_replay(cf_server* s, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
                      const unsigned* kinds, const float* gammas, const int* n_samples, const double* arrival_us,
-                     uint64_t seed, double* latency_us, double* wall_s) {
+                     uint64_t seed, double* latency_us, double* wait_us, double* wall_s) {
     if (!s || n <= 0 || !offsets || !kinds || !arrival_us || !latency_us) return CF_EINVAL;
     std::unordered_map<int64_t, int> which;
     const double start = s->now_us();
@@ -366,8 +395,12 @@ int cf_server_replay(cf_server* s, int n, const char* seqs, const int64_t* offse
         const int k = cf_server_poll(s, ids.data(), (int)ids.size());
         for (int q = 0; q < k; ++q) {
             double lat = -1.0;
+            const auto it = s->reqs.find(ids[q]);
+            const double wait = (it != s->reqs.end() && it->second.state == 2)
+                                    ? it->second.launch_us - it->second.arrival_us : -1.0;
             const int rc = cf_server_result(s, ids[q], nullptr, nullptr, &lat);
             latency_us[which[ids[q]]] = rc ? -1.0 : lat;
+            if (wait_us) wait_us[which[ids[q]]] = rc ? -1.0 : wait;
             ++done;
         }
         if (k == 0) std::this_thread::yield();

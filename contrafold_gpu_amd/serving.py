@@ -108,9 +108,10 @@ class Server:
                 raise TimeoutError("server did not drain")
         return res
 
-    def replay(self, trace, seed=0):
+    def replay(self, trace, seed=0, with_wait=False):
         """Replay a trace in real time inside the library's scheduler loop.
-        Returns (latency_ms array in trace order, wall seconds)."""
+        Returns (latency_ms array in trace order, wall seconds), and with
+        with_wait the queued part of each latency (ms) as a third item."""
         seqs = trace["seqs"]
         n = len(seqs)
         from .engine import encode_batch
@@ -120,6 +121,7 @@ class Server:
         ns = np.array([10 if k == "sample" else 0 for k in trace["kinds"]], np.int32)
         arr = np.ascontiguousarray(np.asarray(trace["arrival_s"], np.float64) * 1e6)
         lat = np.zeros(n, np.float64)
+        wait = np.zeros(n, np.float64)
         wall = _c.c_double()
         check(self.lib.cf_server_replay(self.h, n, buf, offs.ctypes.data_as(_c.POINTER(_c.c_int64)),
                                         mbuf.ctypes.data if mbuf is not None else None,
@@ -127,8 +129,11 @@ class Server:
                                         gammas.ctypes.data_as(_c.POINTER(_c.c_float)),
                                         ns.ctypes.data_as(_c.POINTER(_c.c_int)),
                                         arr.ctypes.data_as(_c.POINTER(_c.c_double)), int(seed),
-                                        lat.ctypes.data_as(_c.POINTER(_c.c_double)), _c.byref(wall)),
+                                        lat.ctypes.data_as(_c.POINTER(_c.c_double)),
+                                        wait.ctypes.data_as(_c.POINTER(_c.c_double)), _c.byref(wall)),
               self.engine.ctx, "cf_server_replay")
+        if with_wait:
+            return lat / 1e3, wall.value, wait / 1e3
         return lat / 1e3, wall.value
 
 
@@ -176,11 +181,21 @@ def benchmark(engine, P, n=12000, fracs=(0.25, 0.5, 0.7, 0.85, 1.0), seed=5, log
         for f in fracs:
             qps = f * plain
             tr = workloads.serving_trace(n, qps, seed=seed)
-            lat, wall = srv.replay(tr)
+            lat, wall, wait = srv.replay(tr, with_wait=True)
             ok = lat >= 0
             row = {"offered_frac": f, "offered_qps": round(qps, 1), "served_folds_per_s": round(n / wall, 1),
                    "served_vs_plain": round(n / wall / plain, 4), "p50_ms": round(float(np.percentile(lat[ok], 50)), 3),
-                   "p99_ms": round(float(np.percentile(lat[ok], 99)), 3), "failed": int((~ok).sum())}
+                   "p99_ms": round(float(np.percentile(lat[ok], 99)), 3), "failed": int((~ok).sum()),
+                   "queued_p50_ms": round(float(np.percentile(wait[ok], 50)), 3)}
+            # latency by length class: the long tail's own fold time bounds p99
+            L = np.asarray(tr["lengths"])
+            by = {}
+            for lo, hi in ((0, 100), (100, 200), (200, 400), (400, 700), (700, 1001)):
+                m = ok & (L > lo) & (L <= hi)
+                if m.any():
+                    by[f"{lo + 1}-{hi}"] = {"n": int(m.sum()), "p50_ms": round(float(np.percentile(lat[m], 50)), 1),
+                                            "p99_ms": round(float(np.percentile(lat[m], 99)), 1)}
+            row["by_length"] = by
             rows.append(row)
             log(f"[serve] {row}")
     finally:

@@ -76,8 +76,10 @@ const char *cf_last_error(const cf_ctx *ctx);
 int cf_set_params(cf_ctx *ctx, const double *tables);
 
 /* Length buckets: sequences with L <= edges[k] (and above edges[k-1]) run in
- * one launch per kernel with threads[k] threads per workgroup.  Defaults are
- * set at cf_create; the autotuner overrides them. */
+ * one launch per kernel with team width threads[k] (a multiple of 64 in
+ * 64..1024): 64 packs four sequences into a workgroup, one wave each; wider
+ * teams give a sequence the whole workgroup (lower latency for long folds).
+ * Defaults are set at cf_create; the autotuner overrides them. */
 int cf_set_tuning(cf_ctx *ctx, const int *edges, const int *threads, int n_buckets);
 
 /* Batch: encode, pack and upload n sequences (gpu.py:482-498). keys: NULL or
@@ -137,10 +139,12 @@ int cf_server_poll(cf_server *s, int64_t *ids, int cap);
 int cf_server_result(cf_server *s, int64_t id, char *structure, float *logz, double *latency_us);
 int cf_server_pending(const cf_server *s);
 /* Replay a recorded arrival trace in real time (arrival_us sorted, relative
- * to the start) and fill per-request latencies (us).  Returns CF_OK. */
+ * to the start) and fill per-request latencies (us) and, if wait_us is not
+ * NULL, the part of each latency spent queued before the request's batch was
+ * launched.  Returns CF_OK. */
 int cf_server_replay(cf_server *s, int n, const char *seqs, const int64_t *offsets, const uint8_t *masks,
                      const unsigned *kinds, const float *gammas, const int *n_samples, const double *arrival_us,
-                     uint64_t seed, double *latency_us, double *wall_s);
+                     uint64_t seed, double *latency_us, double *wait_us, double *wall_s);
 void cf_server_destroy(cf_server *s);
 
 #ifdef __cplusplus
