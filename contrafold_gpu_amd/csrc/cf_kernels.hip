@@ -1015,47 +1015,69 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
             const int lmax = min(CF_CMAX, a - 2);
             const int qbase = b + 2;
             // rows l1 = 0 .. lmax, partners ascending (the reverse of the FCo
-            // order); in row 0 the FM1o source follows the stacking term k = 0
-            for (int l1 = 0; l1 <= lmax; ++l1) {
-                const int ip = a - 1 - l1;
-                uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
-                const int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
-                auto term = [&](int l2) -> float {
-                    const int d = s + l1 + l2 + 2;
-                    const float outside = cf_ld(fco_t, S.dtab[d] + ip) - Z;
-                    float e;
-                    if (l1 + l2 == 0) {
-                        e = outside + bpab;
-                        e = e + stk;
-                        e = e + fcab;
-                    } else {
-                        e = outside + S.JB[rowJ + S.qo[qbase + l2 - 1]];
-                        e = e + S.cs[l1 * 31 + l2];
+            // order); in row 0 the FM1o source follows the stacking term k = 0.
+            // The other sources are gathered 8 at a time (their loads do not
+            // depend on the running sum), then added in order; empty slots
+            // add nothing.
+            if (lmax >= 0) {
+                uint32_t bits = cf_window(S.pmask + S.rs8[a - 1] * S.W, qbase, min(L, qbase + CF_CMAX));
+                if (bits & 1u) {
+                    const float outside = cf_ld(fco_t, S.dtab[s + 2] + a - 1) - Z;
+                    float e = outside + bpab;
+                    e = e + stk;
+                    e = e + fcab;
+                    post = post + ((outside > CF_HALF) ? cf_fexp(e, E) : 0.0f);
+                }
+                bits &= ~1u;
+                if (fm1o_src > CF_HALF) {
+                    float v = fm1o_src + fcab;
+                    v = v + jAx;
+                    v = v + mp;
+                    v = v + bpab;
+                    v = v - Z;
+                    post = post + cf_fexp(v, E);
+                }
+                int l1 = 0, ip = a - 1;
+                bool more = true;
+                while (more) {
+                    int off[8], tag[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        while (bits == 0u && l1 < lmax) {
+                            ++l1;
+                            ip = a - 1 - l1;
+                            bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
+                        }
+                        off[u] = 0;
+                        tag[u] = -1;
+                        if (bits != 0u) {
+                            const int kk = __ffs(bits) - 1;
+                            bits &= bits - 1u;
+                            off[u] = S.dtab[s + l1 + kk + 2] + ip;
+                            tag[u] = (l1 << 5) | kk;
+                        }
+                    }
+                    more = (bits != 0u) || (l1 < lmax);
+                    float o[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) o[u] = cf_ld(fco_t, off[u]);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (tag[u] < 0) continue;
+                        const int r1 = tag[u] >> 5, kk = tag[u] & 31;
+                        const int ipr = a - 1 - r1;
+                        const int rowJ = S.s8[ipr] * 125 + S.s8[ipr + 1] * 5;
+                        const float outside = o[u] - Z;
+                        float e = outside + S.JB[rowJ + S.qo[qbase + kk - 1]];
+                        e = e + S.cs[r1 * 31 + kk];
                         e = e + fcab;
                         e = e + bpab;
                         e = e + jB2;
-                        const float sn = (l1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
-                                       : (l1 == 1) ? (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f)) : 0.0f;
+                        const float sn = (r1 == 0) ? (kk == 1 ? sn01 : 0.0f)
+                                       : (r1 == 1) ? (kk == 0 ? sn10 : (kk == 1 ? sn11 : 0.0f)) : 0.0f;
                         e = e + sn;
+                        post = post + ((outside > CF_HALF) ? cf_fexp(e, E) : 0.0f);
                     }
-                    return (outside > CF_HALF) ? cf_fexp(e, E) : 0.0f;
-                };
-                if (l1 == 0) {
-                    if (bits & 1u) post = post + term(0);
-                    bits &= ~1u;
-                    if (fm1o_src > CF_HALF) {
-                        float v = fm1o_src + fcab;
-                        v = v + jAx;
-                        v = v + mp;
-                        v = v + bpab;
-                        v = v - Z;
-                        post = post + cf_fexp(v, E);
-                    }
-                }
-                while (bits != 0u) {
-                    const int kk = __ffs(bits) - 1;
-                    bits &= bits - 1u;
-                    post = post + term(kk);
                 }
             }
             const float outx = f5o[b + 1] - Z;
@@ -1165,31 +1187,40 @@ __global__ void __launch_bounds__(256) cf_mea_kernel(const CfSeq* __restrict__ s
                         v = post[cf_dbase(L, s - 2) + i + 1] + sc[cf_dbase(L, s - 2) + i + 1];
                         if (v > best) { best = v; t = 3; }
                     }
-                    int64_t ia = cf_dbase(L, 1) + i;
-                    int64_t ib = cf_dbase(L, s - 1) + i + 1;
-                    int d = 1;
-                    for (; d + 7 < s; d += 8) {  // 8 splits' loads in flight, compared in k order
-                        float x[8], y[8];
+                    // splits d = 1 .. s-1 compared in k order; the loads of the
+                    // next 8 splits are in flight while the current 8 compare,
+                    // and past the last split the loads repeat its address and
+                    // the compare is skipped
+                    int ia = (int)cf_dbase(L, 1) + i;
+                    int ib = (int)cf_dbase(L, s - 1) + i + 1;
+                    int dn = 1;
+                    auto load = [&](float (&x)[8], float (&y)[8]) {
 #pragma unroll
                         for (int u = 0; u < 8; ++u) {
-                            x[u] = sc[ia];
-                            y[u] = sc[ib];
-                            ia += L + 1 - (d + u);
-                            ib -= L + 1 - (s - (d + u) - 1);
-                            ib += 1;
+                            x[u] = cf_ld(sc, ia);
+                            y[u] = cf_ld(sc, ib);
+                            const bool adv = dn < s - 1;
+                            ia += adv ? L + 1 - dn : 0;
+                            ib -= adv ? L - (s - dn - 1) : 0;
+                            ++dn;
                         }
+                    };
+                    auto cmp = [&](const float (&x)[8], const float (&y)[8], int d0) {
 #pragma unroll
                         for (int u = 0; u < 8; ++u) {
                             const float w = x[u] + y[u];
-                            if (w > best) { best = w; t = i + d + u + 4; }
+                            if (d0 + u < s && w > best) { best = w; t = i + d0 + u + 4; }
                         }
-                    }
-                    for (; d < s; ++d) {
-                        v = sc[ia] + sc[ib];
-                        if (v > best) { best = v; t = i + d + 4; }
-                        ia += L + 1 - d;
-                        ib -= L + 1 - (s - d - 1);
-                        ib += 1;
+                    };
+                    const int nb = (s - 1 + 7) >> 3;
+                    float xa[8], ya[8], xb[8], yb[8];
+                    load(xa, ya);
+                    for (int k = 0; k < nb; k += 2) {
+                        load(xb, yb);
+                        cmp(xa, ya, 1 + 8 * k);
+                        if (k + 1 >= nb) break;
+                        load(xa, ya);
+                        cmp(xb, yb, 1 + 8 * (k + 1));
                     }
                 }
             }

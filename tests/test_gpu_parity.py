@@ -63,16 +63,22 @@ def same_bits(g, c):
     return both_inf | (g.view(np.uint32) == c.view(np.uint32))
 
 
-@pytest.fixture(scope="module")
-def folded(engine, P, oracle):
-    from contrafold_gpu_amd import CF_BPP
+@pytest.fixture(scope="module", params=[64, 1024], ids=["packed", "wide1024"])
+def folded(P, oracle, request):
+    """The cases folded with one wave per sequence (packed workgroups) and
+    with a 1024-thread workgroup per sequence (the latency-mode sweeps)."""
+    from contrafold_gpu_amd import CF_BPP, Engine
+    eng = Engine()
+    eng.set_params(P)
+    eng.set_tuning([1 << 30], [request.param])
     seqs = [c[0] for c in CASES]
     masks = [c[1] for c in CASES]
-    b = engine.batch(seqs, masks)
+    b = eng.batch(seqs, masks)
     b.run(CF_BPP)
     refs = [oracle.posterior32(s, P, m, tables=True) for s, m in CASES]
     yield b, refs
     b.close()
+    eng.close()
 
 
 @pytest.mark.parametrize("k", range(len(CASES)))
@@ -208,7 +214,7 @@ def test_long_sequence_parity(P, oracle):
     engine.close()
 
 
-@pytest.mark.parametrize("width", [256, 64])
+@pytest.mark.parametrize("width", [256, 1024, 64])
 def test_beyond_lds_ring_parity(P, oracle, width):
     """L = 1100 on a 256-thread workgroup (the FCo ring no longer fits in LDS,
     so the window is read from the global tables) and packed (one wave)."""
