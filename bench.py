@@ -207,6 +207,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", action="store_true")
     args = ap.parse_args()
+    if args.config == "c5" or args.serve:
+        # the server runs one stream per slot; HIP maps streams onto this many
+        # hardware queues (default 4), and streams that share a queue run in
+        # order, so short folds would wait behind long ones.  Must be set
+        # before the HIP runtime starts.
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every fold is independent and this build runs on a single MI355X; "
               "run with --gpus 1", file=sys.stderr)

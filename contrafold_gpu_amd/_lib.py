@@ -53,6 +53,7 @@ SIGNATURES = [
     ("cf_server_poll", _c.c_int, [_vp, _i64p, _c.c_int]),
     ("cf_server_result", _c.c_int, [_vp, _c.c_int64, _c.c_void_p, _f32p, _f64p]),
     ("cf_server_pending", _c.c_int, [_vp]),
+    ("cf_server_stats", _c.c_int, [_vp, _f64p, _c.c_int]),
     ("cf_server_replay", _c.c_int, [_vp, _c.c_int, _c.c_char_p, _i64p, _u8p, _c.POINTER(_c.c_uint), _f32p,
                                     _c.POINTER(_c.c_int), _f64p, _c.c_uint64, _f64p, _f64p, _f64p]),
     ("cf_server_destroy", None, [_vp]),

@@ -69,10 +69,11 @@ int region_get(cf_ctx* c, Region& r, size_t n) {
     if (r.p) c->pool.put(r.p, r.n);
     r = Region();
     void* p = nullptr;
-    hipError_t e = c->pool.get(&p, n);
+    size_t got = 0;
+    hipError_t e = c->pool.get(&p, n, &got);
     if (e != hipSuccess) return fail(c, CF_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
     r.p = p;
-    r.n = DevPool::round(n);
+    r.n = got;
     return CF_OK;
 }
 void region_put(cf_ctx* c, Region& r) {
@@ -95,6 +96,10 @@ int vit_lds(int Lmax) {
 // sequences per workgroup, one wave each, sharing the staged parameter
 // tables.  Wider buckets give the whole workgroup to one sequence.
 constexpr int kPack = 4;
+// sequences per workgroup of the packed inside / outside sweeps (8 per
+// workgroup measured no faster at 200 nt); fewer when LDS is short
+constexpr int kPackIn = 4;
+constexpr int kPackOut = 4;
 constexpr int kLanes = 4;  // streams a multi-bucket batch forks onto
 
 // Workgroup width of the kernels that have no packed layout (MEA, Viterbi,
@@ -114,12 +119,14 @@ int launch_inside(cf_batch* b, const Bucket& bk, hipStream_t st) {
     const int nl = bk.end - bk.begin;
     if (nl <= 0) return CF_OK;
     // The inside sweep reads its single-branch window from the global FC
-    // table (L2-resident): without the LDS ring a workgroup needs ~14 KB of
-    // LDS at 200 nt instead of ~40 KB, which doubles the resident waves
-    // per CU and beats the ring on throughput (54 vs 87 ms for 3000 x 200 nt).
+    // table: without an LDS ring of FC diagonals a packed workgroup needs
+    // ~26 KB of LDS at 200 nt, and more resident waves per CU beat the ring
+    // on throughput (54 vs 87 ms for 3000 x 200 nt).
     const bool pack = bk.threads <= 64;
-    const int G = pack ? kPack : 1;
-    const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, G);
+    // packed: as many sequences per workgroup (up to kPackIn) as LDS allows
+    int G = pack ? kPackIn : 1;
+    CfLds plan = cf_lds_plan(bk.Lmax, false, 0, G);
+    while (plan.total > c->lds_max && G > 1) plan = cf_lds_plan(bk.Lmax, false, 0, G >>= 1);
     if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the inside kernel's LDS plan");
     if (pack)
         hipLaunchKernelGGL((cf_inside_kernel<true>), dim3((nl + G - 1) / G), dim3(64 * G), plan.total, st, b->d_seqs,
@@ -156,17 +163,19 @@ int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st) {
     if (nl <= 0) return CF_OK;
     if (bk.threads <= 64) {
         // packed: no room for a per-sequence FCo ring, the window is read from L2
-        const CfLds plan = cf_lds_plan(bk.Lmax, false, 4, kPack);
+        int G = kPackOut;
+        CfLds plan = cf_lds_plan(bk.Lmax, false, 0, G);
+        while (plan.total > c->lds_max && G > 1) plan = cf_lds_plan(bk.Lmax, false, 0, G >>= 1);
         if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
-        hipLaunchKernelGGL((cf_outside_kernel<false, true>), dim3((nl + kPack - 1) / kPack), dim3(64 * kPack),
+        hipLaunchKernelGGL((cf_outside_kernel<false, true>), dim3((nl + G - 1) / G), dim3(64 * G),
                            plan.total, st, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32,
                            b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
         CF_HIP(c, hipGetLastError(), "outside launch");
         return CF_OK;
     }
-    CfLds plan = cf_lds_plan(bk.Lmax, true, 4);
+    CfLds plan = cf_lds_plan(bk.Lmax, true, 0);
     const bool ring = plan.total <= c->lds_max;
-    if (!ring) plan = cf_lds_plan(bk.Lmax, false, 4);
+    if (!ring) plan = cf_lds_plan(bk.Lmax, false, 0);
     if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
     if (ring)
         hipLaunchKernelGGL((cf_outside_kernel<true, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,

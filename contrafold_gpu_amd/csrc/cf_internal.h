@@ -26,14 +26,18 @@ struct DevPool {
         while (r < n) r <<= 1;
         return r;
     }
-    hipError_t get(void** p, size_t n) {
+    // A free block up to 4x the rounded size is reused (misses cost a
+    // hipMalloc); *got is the block's size, which put() must be given back.
+    hipError_t get(void** p, size_t n, size_t* got) {
         n = round(n ? n : 1);
-        auto it = free_.find(n);
-        if (it != free_.end()) {
+        auto it = free_.lower_bound(n);
+        if (it != free_.end() && it->first <= 4 * n) {
             *p = it->second;
+            *got = it->first;
             free_.erase(it);
             return hipSuccess;
         }
+        *got = n;
         hipError_t e = hipMalloc(p, n);
         if (e != hipSuccess) {  // drop the cache and retry once
             (void)hipGetLastError();
@@ -43,7 +47,7 @@ struct DevPool {
         return e;
     }
     void put(void* p, size_t n) {
-        if (p) free_.emplace(round(n ? n : 1), p);
+        if (p) free_.emplace(n, p);
     }
     void trim() {
         for (auto& kv : free_) (void)hipFree(kv.second);

@@ -61,13 +61,13 @@ __host__ __device__ inline CfLds cf_lds_plan(int Lmax, bool ring, int dbuf_elem,
     o = cf_align(o, 16);
     p.o_pmask = o; o += 5 * p.W * 4;
     o = cf_align(o, 16);
-    p.o_dbuf = o; o += (Lmax + 1 > 128 ? Lmax + 1 : 128) * dbuf_elem;  // also the posterior queue
+    p.o_dbuf = o; o += (Lmax + 1 > 128 ? Lmax + 1 : 128) * dbuf_elem;  // the posterior queue (0: none)
     o = cf_align(o, 16);
     p.o_alist = o; o += 2 * (Lmax + 1) * 2;  // two diagonals' pairable cells
     o = cf_align(o, 16);
     p.o_cnt = o; o += 16;
     p.o_ring = o;
-    if (ring) o += CF_RING * (Lmax + 1) * dbuf_elem;
+    if (ring) o += CF_RING * (Lmax + 1) * 4;
     p.team = cf_align(o, 16);
     p.G = G;
     p.total = p.shared + G * p.team;
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
     extern __shared__ __align__(16) unsigned char smem[];
     const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
     const CfTeam tm = cf_team<PACK>(G);
-    const CfLds plan = cf_lds_plan(Lmax, false, 4, G);
+    const CfLds plan = cf_lds_plan(Lmax, false, 0, G);
     const CfShared S = cf_carve(smem, plan, tm.team);
     cf_stage_params(S, P);
     __syncthreads();
@@ -645,7 +645,7 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
     extern __shared__ __align__(16) unsigned char smem[];
     const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
     const CfTeam tm = cf_team<PACK>(G);
-    const CfLds plan = cf_lds_plan(Lmax, RING, 4, G);
+    const CfLds plan = cf_lds_plan(Lmax, RING, 0, G);
     const CfShared S = cf_carve(smem, plan, tm.team);
     cf_stage_params(S, P);
     __syncthreads();

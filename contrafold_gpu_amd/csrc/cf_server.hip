@@ -48,6 +48,7 @@ struct Slot {
     size_t stage_run_bytes = 0;
     void* out = nullptr;  // pinned: logZ then the kind's payload, copied on the slot's stream
     size_t out_bytes = 0;
+    double launched_us = 0.0;
     cf_batch* batch = nullptr;
     std::vector<int64_t> reqs;
     unsigned kind = 0;
@@ -81,6 +82,11 @@ struct cf_server {
     std::unordered_map<int64_t, Req> reqs;
     std::deque<int64_t> completed;
     int64_t next_id = 0;
+    // Pinned buffers that were outgrown.  hipHostFree waits for the whole
+    // device (every batch in flight), so they are released only at destroy.
+    std::vector<void*> retired_pinned;
+    // statistics (cf_server_stats)
+    double n_batches = 0, n_folds = 0, launch_us = 0, retire_us = 0, run_us = 0, max_pending = 0;
     std::chrono::steady_clock::time_point t0;
     double now_us() const {
         return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
@@ -96,6 +102,7 @@ namespace {
 int retire(cf_server* s, Slot& sl) {
     cf_batch* b = sl.batch;
     const double t = s->now_us();
+    s->run_us += t - sl.launched_us;
     const float* lz = (const float*)sl.out;
     const char* pay = (const char*)sl.out + sizeof(float) * (size_t)b->n;
     int64_t toff = 0;
@@ -126,7 +133,22 @@ int retire(cf_server* s, Slot& sl) {
     sl.reqs.clear();
     s->in_use -= sl.bytes;
     sl.bytes = 0;
+    s->retire_us += s->now_us() - t;
     return CF_OK;
+}
+
+// Grows a slot's pinned buffer to at least `need` bytes (powers of two from
+// 1 MiB); the old buffer is parked until the server is destroyed.
+bool grow_pinned(cf_server* s, void** p, size_t* bytes, size_t need) {
+    if (*bytes >= need) return true;
+    size_t cap = (size_t)1 << 20;
+    while (cap < need) cap <<= 1;
+    void* q = nullptr;
+    if (hipHostMalloc(&q, cap) != hipSuccess) return false;
+    if (*p) s->retired_pinned.push_back(*p);
+    *p = q;
+    *bytes = cap;
+    return true;
 }
 
 // Workgroup width for a small batch whose longest fold has length Lmax: each
@@ -139,6 +161,8 @@ constexpr int kLongClass = 3;
 
 int launch(cf_server* s, Slot& sl) {
     cf_ctx* c = s->ctx;
+    const double t_start = s->now_us();
+    s->max_pending = std::max(s->max_pending, (double)s->pending.size());
     while (!s->pending.empty() && s->reqs[s->pending.front()].state != 0) s->pending.pop_front();
     if (s->pending.empty()) return 0;
     // the oldest request decides the batch kind and length class, unless it is
@@ -217,20 +241,16 @@ int launch(cf_server* s, Slot& sl) {
     for (int k = 0; k < n; ++k) lmax = std::max(lmax, (int)(offs[k + 1] - offs[k]));
     const int width = (n <= 4 * c->n_cu) ? serve_width(lmax) : 0;
     cf_batch* b = nullptr;
-    int rc = cfi::batch_create(c, n, seqs.data(), offs.data(), any_mask ? masks.data() : nullptr, keys.data(),
+    int rc = CF_OK;
+    // the slot's previous batch has retired, so its pinned buffers are idle
+    if (!grow_pinned(s, &sl.stage_in, &sl.stage_in_bytes, cfi::batch_staging_bytes(n, offs[n] + 2 * (int64_t)n)))
+        rc = cfi::fail(c, CF_ENOMEM, "pinned staging");
+    if (!rc)
+        rc = cfi::batch_create(c, n, seqs.data(), offs.data(), any_mask ? masks.data() : nullptr, keys.data(),
                                sl.stream, &sl.stage_in, &sl.stage_in_bytes, false, &b, width);
     if (!rc) {
         const size_t need = sizeof(float) * (cfi::kMaxGamma + (size_t)n) + sizeof(int64_t) * (size_t)n + 4096;
-        if (sl.stage_run_bytes < need) {
-            if (sl.stage_run) {
-                (void)hipStreamSynchronize(sl.stream);
-                (void)hipHostFree(sl.stage_run);
-            }
-            sl.stage_run = nullptr;
-            sl.stage_run_bytes = 0;
-            if (hipHostMalloc(&sl.stage_run, need) != hipSuccess) rc = cfi::fail(c, CF_ENOMEM, "pinned staging");
-            else sl.stage_run_bytes = need;
-        }
+        if (!grow_pinned(s, &sl.stage_run, &sl.stage_run_bytes, need)) rc = cfi::fail(c, CF_ENOMEM, "pinned staging");
     }
     if (!rc)
         rc = cfi::batch_enqueue(b, kind_flags(kind), &gamma, 1, nsamp > 0 ? nsamp : 1, 0ull, nullptr,
@@ -243,14 +263,7 @@ int launch(cf_server* s, Slot& sl) {
         else if (kind == CF_SAMPLE) src = b->samp_out, pay = (size_t)b->n_samples * b->total_chars;
         else if (kind == CF_BPP) src = b->POST, pay = sizeof(float) * (size_t)b->total_T;
         const size_t need = sizeof(float) * (size_t)n + pay;
-        if (sl.out_bytes < need) {
-            if (sl.out) (void)hipHostFree(sl.out);  // the slot's stream is idle: its last batch was retired
-            sl.out = nullptr;
-            sl.out_bytes = 0;
-            const size_t cap = std::max(need, (size_t)1 << 20);
-            if (hipHostMalloc(&sl.out, cap) != hipSuccess) rc = cfi::fail(c, CF_ENOMEM, "pinned result buffer");
-            else sl.out_bytes = cap;
-        }
+        if (!grow_pinned(s, &sl.out, &sl.out_bytes, need)) rc = cfi::fail(c, CF_ENOMEM, "pinned result buffer");
         if (!rc && hipMemcpyAsync(sl.out, b->logz, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, sl.stream) !=
                        hipSuccess)
             rc = cfi::fail(c, CF_EDEVICE, "result copy");
@@ -276,6 +289,10 @@ int launch(cf_server* s, Slot& sl) {
     sl.long_class = lclass >= kLongClass;
     sl.bytes = bytes;
     s->in_use += bytes;
+    sl.launched_us = s->now_us();
+    s->n_batches += 1;
+    s->n_folds += n;
+    s->launch_us += sl.launched_us - t_start;
     return n;
 }
 
@@ -363,6 +380,13 @@ int cf_server_result(cf_server* s, int64_t id, char* structure, float* logz, dou
     return CF_OK;
 }
 
+int cf_server_stats(const cf_server* s, double* out, int n) {
+    if (!s || !out || n < 0) return CF_EINVAL;
+    const double v[6] = {s->n_batches, s->n_folds, s->launch_us, s->retire_us, s->run_us, s->max_pending};
+    for (int k = 0; k < n && k < 6; ++k) out[k] = v[k];
+    return std::min(n, 6);
+}
+
 int cf_server_pending(const cf_server* s) {
     if (!s) return 0;
     int n = (int)s->pending.size();
@@ -420,6 +444,7 @@ void cf_server_destroy(cf_server* s) {
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
+    for (void* p : s->retired_pinned) (void)hipHostFree(p);
     delete s;
 }
 

@@ -32,6 +32,15 @@ def request_key(seq, mask=None, seed=0):
 
 
 class Server:
+    """Continuous-batching fold server over one engine (cf_server_*).
+
+    Each of the n_streams slots runs its batches on its own HIP stream.  HIP
+    maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default) and a
+    queue runs its work in order, so set GPU_MAX_HW_QUEUES >= n_streams in
+    the environment before the process first uses the GPU; otherwise short
+    folds wait behind long ones that share their queue.
+    """
+
     def __init__(self, engine, hbm_budget=64 << 30, max_batch=512, n_streams=8):
         self.engine = engine
         self.lib = engine.lib
@@ -77,6 +86,18 @@ class Server:
 
     def pending(self):
         return int(self.lib.cf_server_pending(self.h))
+
+    STATS = ("batches", "folds", "host_launch_ms", "host_retire_ms", "batch_residence_ms", "max_queue")
+
+    def stats(self):
+        """Scheduler counters since creation (cf_server_stats)."""
+        out = np.zeros(len(self.STATS), np.float64)
+        n = self.lib.cf_server_stats(self.h, out.ctypes.data_as(_c.POINTER(_c.c_double)), len(out))
+        check(min(n, 0), self.engine.ctx, "cf_server_stats")
+        d = dict(zip(self.STATS, out.tolist()))
+        for k in ("host_launch_ms", "host_retire_ms", "batch_residence_ms"):
+            d[k] /= 1e3
+        return d
 
     def result(self, rid):
         kind, L, ns = self.meta.pop(rid)

@@ -138,6 +138,11 @@ int cf_server_poll(cf_server *s, int64_t *ids, int cap);
  * latency_us: completion - arrival. */
 int cf_server_result(cf_server *s, int64_t id, char *structure, float *logz, double *latency_us);
 int cf_server_pending(const cf_server *s);
+/* Counters since cf_server_create, up to n doubles in this order: batches
+ * launched, folds launched, host microseconds spent launching, host
+ * microseconds spent retiring, summed batch residence (launch to retirement,
+ * microseconds), largest queue length seen.  Returns how many were written. */
+int cf_server_stats(const cf_server *s, double *out, int n);
 /* Replay a recorded arrival trace in real time (arrival_us sorted, relative
  * to the start) and fill per-request latencies (us) and, if wait_us is not
  * NULL, the part of each latency spent queued before the request's batch was
