@@ -211,8 +211,13 @@ def main():
         # the server runs one stream per slot; HIP maps streams onto this many
         # hardware queues (default 4), and streams that share a queue run in
         # order, so short folds would wait behind long ones.  Must be set
-        # before the HIP runtime starts.
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+        # before the HIP runtime starts (32 is the largest value accepted here).
+        try:
+            cur = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+        except ValueError:
+            cur = 0
+        if cur < 32:
+            os.environ["GPU_MAX_HW_QUEUES"] = "32"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every fold is independent and this build runs on a single MI355X; "
               "run with --gpus 1", file=sys.stderr)

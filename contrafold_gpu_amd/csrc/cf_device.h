@@ -130,7 +130,11 @@ __device__ __forceinline__ float cf_lse(float a, float b, const float4* __restri
     r = r * d;
     r = r + c.w;
     const float res = lo + r;
-    return (lo < CF_HALF || d >= 11.8624794162f) ? hi : res;
+    // The reference also returns hi when lo < -5e29.  Values below that are
+    // exactly NEG = -1e30 (scores never come near): then either hi is finite
+    // and d >= 11.86 anyway, or hi = lo = NEG, d = 0 and lo + r rounds back to
+    // NEG (|r| < 1 is far below half an ulp of 1e30), so one test suffices.
+    return (d >= 11.8624794162f) ? hi : res;
 }
 
 // CONTRAfold Fast_Exp for RealT=float (cpu.py:560-580), branch-free except

@@ -492,7 +492,11 @@ __device__ __forceinline__ float cf_lse_u(float a, float b) {
     r = r * d;
     r = r + cf_lse_coef[seg][3];
     const float res = lo + r;
-    return (lo < CF_HALF || d >= 11.8624794162f) ? hi : res;
+    // The reference also returns hi when lo < -5e29.  Values below that are
+    // exactly NEG = -1e30 (scores never come near): then either hi is finite
+    // and d >= 11.86 anyway, or hi = lo = NEG, d = 0 and lo + r rounds back to
+    // NEG (|r| < 1 is far below half an ulp of 1e30), so one test suffices.
+    return (d >= 11.8624794162f) ? hi : res;
 }
 
 __host__ __device__ inline int cf_ext_wave_bytes(int Lmax) { return cf_align((Lmax + 2) * 4 + 2 * (Lmax + 3), 16); }

@@ -301,7 +301,7 @@ int launch(cf_server* s, Slot& sl) {
 extern "C" {
 
 int cf_server_create(cf_ctx* c, int64_t budget, int max_batch, int n_streams, cf_server** out) {
-    if (!c || !out || budget <= 0 || max_batch <= 0 || n_streams <= 0 || n_streams > 16)
+    if (!c || !out || budget <= 0 || max_batch <= 0 || n_streams <= 0 || n_streams > 32)
         return cfi::fail(c, CF_EINVAL, "bad server arguments");
     *out = nullptr;
     if (hipSetDevice(c->device) != hipSuccess) return cfi::fail(c, CF_EDEVICE, "hipSetDevice");

@@ -122,6 +122,10 @@ int64_t cf_fold_bytes(int L, unsigned flags, int n_gamma, int n_samples);
  * Requests are admitted while their O(L^2) table bytes fit the HBM budget;
  * admitted requests are launched as batches on a pool of streams, so work
  * that arrives while others run overlaps with it on the device. */
+/* n_streams (1..32) slots, each with its own stream.  HIP maps streams onto
+ * GPU_MAX_HW_QUEUES hardware queues (default 4) and a queue runs in order, so
+ * set that environment variable >= n_streams before the process first uses the
+ * GPU, or short folds wait behind long ones that share their queue. */
 int cf_server_create(cf_ctx *ctx, int64_t hbm_budget, int max_batch, int n_streams, cf_server **out);
 /* kind: CF_MEA, CF_BPP, CF_SAMPLE or CF_LOGZ; returns a request id (>= 0) or
  * a negative status.  seed: the request's 64-bit sampling-stream key (the same
