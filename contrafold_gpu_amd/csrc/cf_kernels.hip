@@ -1695,7 +1695,9 @@ __device__ __forceinline__ int cf_scan_pick(float p, bool valid, float r, float&
     return -1;
 }
 
-__global__ void __launch_bounds__(256) cf_sample_kernel(
+// Eight waves per SIMD (the register cap spills ~112 bytes per lane of cold
+// state) beat five unspilled ones: 31 vs 41 ms for 10,000 x 200 nt x 100.
+__global__ void __launch_bounds__(256, 8) cf_sample_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ sidx, int nseq, int nsamp, uint64_t seed,
     const uint64_t* __restrict__ keys, const uint8_t* __restrict__ codes, const uint8_t* __restrict__ masks,
     const CfParams<float>* __restrict__ P, const float* __restrict__ FC, const float* __restrict__ FM,

@@ -2,6 +2,8 @@
 FP64 finite differences, graph replay, launch-shape invariance, serving."""
 import math
 
+import time
+
 import numpy as np
 import pytest
 
@@ -261,6 +263,30 @@ def test_server_admission_budget(engine, P):
     srv.close()
     assert done[big] is None
     assert all(done[i] is not None for i in ids)
+
+
+def test_server_stats_and_long_folds(engine, P):
+    """Long folds (> 320 nt) hold at most half the slots; short requests
+    submitted behind them still complete, and the counters add up."""
+    from contrafold_gpu_amd.serving import Server
+    srv = Server(engine, n_streams=4, max_batch=4)
+    rng = np.random.default_rng(21)
+    longs = [srv.submit(rand_seq(rng, 600)) for _ in range(12)]
+    shorts = [srv.submit(rand_seq(rng, 40)) for _ in range(12)]
+    done = {}
+    t0 = time.time()
+    while len(done) < 24 and time.time() - t0 < 60:
+        for rid in srv.poll():
+            done[rid] = srv.result(rid)
+    st = srv.stats()
+    srv.close()
+    assert len(done) == 24
+    assert all(len(done[i]["structure"]) == 600 for i in longs)
+    assert all(len(done[i]["structure"]) == 40 for i in shorts)
+    assert st["folds"] == 24 and st["batches"] >= 6  # max_batch 4: at least 3 long + 3 short batches
+    assert st["max_queue"] >= 1 and st["host_launch_ms"] >= 0 and st["batch_residence_ms"] > 0
+    # the short requests did not wait for every long batch: some finished first
+    assert min(done[i]["latency_ms"] for i in shorts) < max(done[i]["latency_ms"] for i in longs)
 
 
 def test_autotune_installs_fastest(engine, P, tmp_path, monkeypatch):
