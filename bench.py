@@ -155,6 +155,26 @@ def run_config(name, args):
                     "max_abs_err": float(max(err)),
                     "max_rel_err_nonzero": float(max([e / abs(fd) for e, (_, fd) in zip(err, rows) if abs(fd) > 1e-3]
                                                      or [0.0]))})
+    elif name == "latency":
+        # single-fold latency floor: one uniform-ACGU sequence alone on the GPU,
+        # one wave (packed layout) vs a 1,024-thread workgroup, device ms per kind
+        rng = np.random.default_rng(1)
+        rows = []
+        for L in (100, 200, 400, 600, 1000):
+            seq = "".join(rng.choice(list("ACGU"), L))
+            for w in (64, 1024):
+                eng.set_tuning([1 << 30], [w])
+                with eng.batch([seq]) as b:
+                    row = {"L": L, "threads": w}
+                    for flags, kw, kind in ((cf.CF_MEA, dict(gammas=[6.0]), "mea"), (cf.CF_BPP, {}, "bpp"),
+                                            (cf.CF_SAMPLE, dict(n_samples=10), "sample10")):
+                        b.run(flags, recompute=True, **kw)
+                        b.run(flags, recompute=True, **kw)
+                        row[f"{kind}_ms"] = round(b.last_ms(), 2)
+                    rows.append(row)
+                    log(f"[latency] {row}")
+        eng.set_tuning(*autotune.DEFAULT)
+        out["rows"] = rows
     elif name == "c5":
         from contrafold_gpu_amd import serving
         out.update(serving.benchmark(eng, P, n=args.serve_n, log=log))
@@ -194,7 +214,7 @@ def published_workloads(cf, workloads, P, log, reps=3):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "latency"],
                     help="c2 (default) is the bench line; c1/c3/c4/c5 print their own measurements")
     ap.add_argument("--check", type=int, default=64, help="sequences the oracle re-folds for c1/c4")
     ap.add_argument("--serve-n", type=int, default=12000)
@@ -206,6 +226,9 @@ def main():
     ap.add_argument("--samples", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="for rocprofv3 counter passes: a copy-like calibration kernel of known size "
+                         "(torch add: 256 MiB read, 256 MiB written), then exactly one step of the workload")
     args = ap.parse_args()
     if args.config == "c5" or args.serve:
         # the server runs one stream per slot; HIP maps streams onto this many
@@ -226,6 +249,16 @@ def main():
         run_config(args.config, args)
         return
 
+    calib = None
+    if args.probe:
+        # the calibration kernel runs before the engine starts: under
+        # rocprofv3 --pmc, torch.ones crashed when torch started its GPU
+        # context after the engine had
+        import torch
+        x = torch.ones(64 << 20, device="cuda")
+        y = x + 1.0
+        torch.cuda.synchronize()
+        calib = float(y[0])
     import contrafold_gpu_amd as cf
     from contrafold_gpu_amd import workcount, workloads
     P = cf.load()
@@ -235,6 +268,10 @@ def main():
     eng.set_params(P)
     batch = eng.batch(seqs, masks)
     flags = cf.CF_BPP | cf.CF_SAMPLE
+    if args.probe:
+        batch.run(flags, n_samples=args.samples, seed=1, recompute=True)
+        print(json.dumps({"probe": f"{args.n} x {args.length} nt, one step", "calibration": calib}), flush=True)
+        return
     log(f"[bench] batch resident: {batch.device_bytes() / 2**20:.0f} MiB before run buffers")
     for w in range(args.warmup):
         t = time.perf_counter()
