@@ -373,43 +373,52 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
             // Rows p >= i+2: candidates (p ascending, q descending) gathered 8
             // at a time -- their reads do not depend on acc -- then folded in
             // order; exhausted slots hold NEG, which leaves acc unchanged.
-            int p = i + 2, qlo = 0, rowJ = 0;
+            // Per row, LDS pointers are rebased so a candidate k (q = qlo + k)
+            // indexes them directly: dtab[q - p - 2], qinfo[q], cs[l1*31 + j - q].
+            int p = i + 2, rowJ = 0;
             uint32_t bits = 0u;
-            if (p <= pend) {
-                qlo = max(p + 2, j - CF_CMAX + 2);
+            const int* dtr = S.dtab;
+            const int* qir = S.qinfo;
+            const float* csr = S.cs;
+            auto row = [&]() {
+                const int qlo = max(p + 2, j - CF_CMAX + (p - i));
                 rowJ = S.s8[p + 1] * 25 + S.s8[p];
                 bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
-            }
+                dtr = S.dtab + (qlo - p - 2);
+                qir = S.qinfo + qlo;
+                csr = S.cs + ((p - i) * 31 + j - qlo);
+            };
+            if (p <= pend) row();
             bool more = (p <= pend);
+            // The FC loads of a batch are consumed only in its fold, so all 8 are
+            // in flight together.  An empty slot reads cell (0, 0), which the
+            // first step set to NEG, so its term is NEG.
             while (more) {
-                float v[8];
+                float fv[8], ev[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     while (bits == 0u && p < pend) {
                         ++p;
-                        qlo = max(p + 2, j - CF_CMAX + (p - i));
-                        rowJ = S.s8[p + 1] * 25 + S.s8[p];
-                        bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
+                        row();
                     }
-                    float val = CF_NEG;
+                    int off = 0;
+                    float e = 0.0f;
                     if (bits != 0u) {
                         const int k = 31 - __clz(bits);
                         bits &= ~(1u << k);
-                        const int q = qlo + k;
-                        const int l1 = p - i, l2 = j - q;
-                        const float fc = cf_ld(fc_t, S.dtab[d - l1 - l2 - 2] + p + 1);
-                        const float2 bj = S.BJ[(S.qinfo[q] & 0xffff) + rowJ];
-                        float e = S.cs[l1 * 31 + l2] + bj.y;
+                        off = dtr[k] + p + 1;
+                        const float2 bj = S.BJ[(qir[k] & 0xffff) + rowJ];
+                        e = csr[-k] + bj.y;
                         e = e + jB;
                         e = e + bj.x;
                         e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
-                        val = (fc > CF_HALF) ? e + fc : CF_NEG;
                     }
-                    v[u] = val;
+                    fv[u] = cf_ld(fc_t, off);
+                    ev[u] = e;
                 }
                 more = (bits != 0u) || (p < pend);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) acc = cf_lse(acc, v[u], C);
+                for (int u = 0; u < 8; ++u) acc = cf_lse(acc, (fv[u] > CF_HALF) ? ev[u] + fv[u] : CF_NEG, C);
             }
             const float fm2 = fm2_t[db + i];
             if (fm2 > CF_HALF) {
@@ -891,9 +900,17 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
                 int ip = a - 1 - l1;
                 uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
                 int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
+                // per row, candidate l2 indexes dtab[d + l1 + 2 + l2] and cs[l1*31 + l2]
+                const int* dtr = S.dtab + (d + l1 + 2);
+                const float* csr = S.cs + l1 * 31;
+                const int16_t* qor = S.qo + (qbase - 1);
+                // Without the ring the FCo loads of a batch are consumed only in
+                // its fold, so all 8 are in flight together; an empty slot reads
+                // cell (0, L), set to NEG by the first step, and its term is NEG.
+                const int off_neg = S.dtab[L];
                 bool more = true;
                 while (more) {
-                    float v[8];
+                    float fv[8], ev[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         while (bits == 0u && l1 > 2) {
@@ -901,24 +918,47 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
                             ip = a - 1 - l1;
                             bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
                             rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
+                            dtr = S.dtab + (d + l1 + 2);
+                            csr = S.cs + l1 * 31;
                         }
-                        float val = CF_NEG;
-                        if (bits != 0u) {
-                            const int l2 = 31 - __clz(bits);
-                            bits &= ~(1u << l2);
-                            const int dd = d + l1 + l2 + 2;
-                            const float fco = RING ? ring[(dd % CF_RING) * RS + ip] : cf_ld(fco_t, S.dtab[dd] + ip);
-                            float e = S.cs[l1 * 31 + l2] + bpab;
-                            e = e + S.JB[rowJ + S.qo[qbase + l2 - 1]];
-                            e = e + jB2;
-                            e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
-                            val = (fco > CF_HALF) ? fco + e : CF_NEG;
+                        float e = 0.0f;
+                        if constexpr (RING) {
+                            float val = CF_NEG;
+                            if (bits != 0u) {
+                                const int l2 = 31 - __clz(bits);
+                                bits &= ~(1u << l2);
+                                const int dd = d + l1 + l2 + 2;
+                                const float fco = ring[(dd % CF_RING) * RS + ip];
+                                e = csr[l2] + bpab;
+                                e = e + S.JB[rowJ + qor[l2]];
+                                e = e + jB2;
+                                e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
+                                val = (fco > CF_HALF) ? fco + e : CF_NEG;
+                            }
+                            fv[u] = val;
+                        } else {
+                            int off = off_neg;
+                            if (bits != 0u) {
+                                const int l2 = 31 - __clz(bits);
+                                bits &= ~(1u << l2);
+                                off = dtr[l2] + ip;
+                                e = csr[l2] + bpab;
+                                e = e + S.JB[rowJ + qor[l2]];
+                                e = e + jB2;
+                                e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
+                            }
+                            fv[u] = cf_ld(fco_t, off);
+                            ev[u] = e;
                         }
-                        v[u] = val;
                     }
                     more = (bits != 0u) || (l1 > 2);
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) acc = cf_lse(acc, v[u], C);
+                    for (int u = 0; u < 8; ++u) {
+                        if constexpr (RING)
+                            acc = cf_lse(acc, fv[u], C);
+                        else
+                            acc = cf_lse(acc, (fv[u] > CF_HALF) ? fv[u] + ev[u] : CF_NEG, C);
+                    }
                 }
             }
             for (int l1 = min(lmax, 1); l1 >= 0; --l1) {
