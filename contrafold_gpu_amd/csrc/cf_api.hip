@@ -87,9 +87,7 @@ __global__ void cf_fill_f64(double* p, int64_t n, double v) {
 }
 
 int mea_lds(int Lmax) { return (int)align_up((size_t)(Lmax + 2) * 4 + 2 * (Lmax + 2) + 16, 16) + 2 * (Lmax + 2) * 4 + 16; }
-int vit_lds(int Lmax) {
-    return (int)align_up((size_t)(Lmax + 2) * 8 + 8 * 8 + 2 * (Lmax + 3) + 16, 16) + (4 * Lmax + 16) * 4 + 16;
-}
+int vit_lds(int Lmax) { return cf_vit_plan(Lmax).total; }
 
 // ---------------------------------------------------------------- stages
 // A bucket whose tuned width is one wave (64 threads) runs packed: kPack

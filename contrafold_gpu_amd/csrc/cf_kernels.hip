@@ -1321,6 +1321,36 @@ __device__ __forceinline__ void cf_acc(double& acc, double v) {
     if (LSE) acc = cf_lse64(acc, v);
     else if (v > acc) acc = v;
 }
+// LDS of the FP64 sweep: F5 chain, reduction partials, codes, traceback stack,
+// the diagonal's FM2 values, partner masks, the pairable-cell list and the
+// FP64 cs / JB / bp tables.
+struct CfVitLds {
+    int W, o_red, o_s8, o_fo8, o_rs8, o_st, o_fm2, o_pmask, o_alist, o_cnt, o_cs, o_jb, o_bp, total;
+};
+__host__ __device__ inline CfVitLds cf_vit_plan(int Lmax) {
+    CfVitLds p;
+    p.W = ((Lmax + 2 + 31) >> 5) + 1;
+    int o = (Lmax + 2) * 8;  // f5 at 0
+    p.o_red = o; o += 8 * 8;
+    p.o_s8 = o; o += Lmax + 3;
+    p.o_fo8 = o; o += Lmax + 3;
+    p.o_rs8 = o; o += Lmax + 3;
+    o = cf_align(o, 16);
+    p.o_st = o; o += (4 * Lmax + 16) * 4;
+    o = cf_align(o, 16);
+    p.o_fm2 = o; o += (Lmax + 1) * 8;
+    p.o_pmask = o; o += 5 * p.W * 4;
+    o = cf_align(o, 16);
+    p.o_alist = o; o += (Lmax + 1) * 2;
+    o = cf_align(o, 16);
+    p.o_cnt = o; o += 16;
+    p.o_cs = o; o += 961 * 8;
+    p.o_jb = o; o += 625 * 8;
+    p.o_bp = o; o += 25 * 8;
+    p.total = cf_align(o, 16);
+    return p;
+}
+
 template <bool LSE>
 __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
                                                          const uint8_t* __restrict__ codes,
@@ -1331,11 +1361,21 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                                                          uint8_t* __restrict__ out, const int64_t* __restrict__ out_off,
                                                          int Lmax) {
     extern __shared__ __align__(16) unsigned char smem[];
+    const CfVitLds lp = cf_vit_plan(Lmax);
     double* f5 = (double*)smem;                          // L+2
-    double* red = f5 + (Lmax + 2);                       // 8 wave partials
-    int8_t* s8 = (int8_t*)(red + 8);                     // L+3
-    int8_t* fo8 = s8 + (Lmax + 3);                       // L+3
-    uint32_t* st = (uint32_t*)(((uintptr_t)(fo8 + Lmax + 3) + 15) & ~(uintptr_t)15);  // 4L+16
+    double* red = (double*)(smem + lp.o_red);            // 8 wave partials
+    int8_t* s8 = (int8_t*)(smem + lp.o_s8);              // L+3
+    int8_t* fo8 = (int8_t*)(smem + lp.o_fo8);            // L+3
+    int8_t* rs8 = (int8_t*)(smem + lp.o_rs8);            // L+3: pmask row of each base (4 = pairs with nothing)
+    uint32_t* st = (uint32_t*)(smem + lp.o_st);          // 4L+16
+    double* fm2b = (double*)(smem + lp.o_fm2);           // FM2 of the current diagonal
+    uint32_t* pmask = (uint32_t*)(smem + lp.o_pmask);    // 5 x W
+    int16_t* alist = (int16_t*)(smem + lp.o_alist);      // pairable cells of the current diagonal
+    int* cnt = (int*)(smem + lp.o_cnt);
+    double* cs64 = (double*)(smem + lp.o_cs);
+    double* jb64 = (double*)(smem + lp.o_jb);
+    double* bp64 = (double*)(smem + lp.o_bp);
+    const int W = lp.W;
     const int bseq = order[blockIdx.x];
     const CfSeq sd = seqs[bseq];
     const int L = sd.L;
@@ -1344,7 +1384,24 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
         s8[q] = (int8_t)codes[sd.coff + q];
         fo8[q] = (masks != nullptr && masks[sd.coff + q] != 0) ? 1 : 0;
     }
-    if (tid == 0) { s8[L + 2] = 4; fo8[L + 2] = 0; }
+    for (int k = tid; k < 961; k += NT) cs64[k] = P->t[CF_O_CS + k];
+    for (int k = tid; k < 625; k += NT) jb64[k] = P->JB[k];
+    for (int k = tid; k < 25; k += NT) bp64[k] = P->t[CF_O_BP + k];
+    if (tid == 0) { s8[L + 2] = 4; fo8[L + 2] = 0; cnt[0] = 0; cnt[1] = 0; }
+    __syncthreads();
+    for (int q = tid; q < L + 3; q += NT) rs8[q] = (fo8[q] || s8[q] == 4) ? 4 : s8[q];
+    for (int w = tid; w < W; w += NT) {
+        uint32_t m[5] = {0u, 0u, 0u, 0u, 0u};
+        for (int bit = 0; bit < 32; ++bit) {
+            const int q = w * 32 + bit;
+            if (q > L + 1) break;
+            const int c = s8[q];
+            if (fo8[q] || c == 4) continue;
+            for (int x = 0; x < 4; ++x)
+                if (cf_canon(x, c)) m[x] |= 1u << bit;
+        }
+        for (int x = 0; x < 5; ++x) pmask[x * W + w] = m[x];
+    }
     __syncthreads();
     const double* __restrict__ T = P->t;
     const double mb = T[CF_O_MB], mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
@@ -1353,6 +1410,10 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
     double* __restrict__ fm1_t = FM1 + sd.toff;
     auto allow = [&](int a, int b) -> bool { return cf_canon(s8[a], s8[b]) && !fo8[a] && !fo8[b]; };
 
+    // Per diagonal: (1) the multiloop split FM2, FM1 and FM of every cell (they
+    // read diagonals < s only) and the pairable cells into a list; (2) FC of the
+    // listed cells, the single-branch window walked over the set bits of the
+    // partner masks in the reference's (p ascending, q descending) order.
     for (int s = 0; s <= L; ++s) {
         const int64_t db = cf_dbase(L, s);
         for (int i = tid; i <= L - s; i += NT) {
@@ -1385,48 +1446,6 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                     ib += 1;
                 }
             }
-            double fcv = CF_NEG64;
-            if (inner && allow(i, j + 1)) {
-                const int si = s8[i], sj1 = s8[j + 1], si1 = s8[i + 1], sj = s8[j];
-                const double jB = P->JB[cf_i4(si, sj1, si1, sj)];
-                double best = jB + T[CF_O_HP + (s < 30 ? s : 30)];
-                const int pmax = min(i + CF_CMAX, j);
-                for (int p = i; p <= pmax; ++p) {
-                    const int l1 = p - i;
-                    const int qmin = max(p + 2, p - i + j - CF_CMAX);
-                    for (int q = j; q >= qmin; --q) {
-                        if (!allow(p + 1, q)) continue;
-                        const double fc = fc_t[cf_tri(L, p + 1, q - 1)];
-                        if (!(fc > CF_HALF64)) continue;
-                        const int l2 = j - q;
-                        double e;
-                        if (l1 == 0 && l2 == 0) {
-                            e = T[CF_O_BP + cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
-                        } else {
-                            const double jB2 = P->JB[cf_i4(s8[q], s8[p + 1], s8[q + 1], s8[p])];
-                            double sn = 0.0;
-                            if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
-                            else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
-                            else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
-                            e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s8[p + 1], s8[q])];
-                            e = e + jB;
-                            e = e + jB2;
-                            e = e + sn;
-                        }
-                        cf_acc<LSE>(best, e + fc);
-                    }
-                }
-                if (fm2 > CF_HALF64) {
-                    double jA = T[CF_O_HC + cf_i2(si, sj1)];
-                    if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
-                    if (j > 0) jA = jA + T[CF_O_DR + cf_i3(si, sj1, sj)];
-                    double v = fm2 + jA;
-                    v = v + mp;
-                    v = v + mb;
-                    cf_acc<LSE>(best, v);
-                }
-                fcv = best;
-            }
             double fm1 = CF_NEG64, fmv = CF_NEG64;
             if (inner && s >= 2) {
                 const int si1 = s8[i + 1], sj = s8[j];
@@ -1438,7 +1457,7 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                         if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, s8[i])];
                         double v = fcin + jA;
                         v = v + mp;
-                        v = v + T[CF_O_BP + cf_i2(si1, sj)];
+                        v = v + bp64[cf_i2(si1, sj)];
                         cf_acc<LSE>(fm1, v);
                     }
                 }
@@ -1449,9 +1468,65 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                 if (fml > CF_HALF64) cf_acc<LSE>(fmv, fml + mu);
                 if (fm1 > CF_HALF64) cf_acc<LSE>(fmv, fm1);
             }
-            fc_t[db + i] = fcv;
             fm1_t[db + i] = fm1;
             fm_t[db + i] = fmv;
+            if (inner && allow(i, j + 1)) {
+                fm2b[i] = fm2;
+                alist[atomicAdd(&cnt[s & 1], 1)] = (int16_t)i;
+            } else {
+                fc_t[db + i] = CF_NEG64;
+            }
+        }
+        __syncthreads();
+        const int nal = cnt[s & 1];
+        if (tid == 0) cnt[(s + 1) & 1] = 0;
+        for (int idx = tid; idx < nal; idx += NT) {
+            const int i = alist[idx];
+            const int j = i + s;
+            const int si = s8[i], sj1 = s8[j + 1], si1 = s8[i + 1], sj = s8[j];
+            const double jB = jb64[cf_i4(si, sj1, si1, sj)];
+            double best = jB + T[CF_O_HP + (s < 30 ? s : 30)];
+            const int pmax = min(i + CF_CMAX, j);
+            for (int p = i; p <= pmax; ++p) {
+                const int l1 = p - i;
+                const int qmin = max(p + 2, p - i + j - CF_CMAX);
+                uint32_t bits = cf_window(pmask + rs8[p + 1] * W, qmin, j);
+                const int sp1 = s8[p + 1], spp = s8[p];
+                while (bits != 0u) {
+                    const int k = 31 - __clz(bits);
+                    bits &= ~(1u << k);
+                    const int q = qmin + k;
+                    const double fc = fc_t[cf_tri(L, p + 1, q - 1)];
+                    if (!(fc > CF_HALF64)) continue;
+                    const int l2 = j - q;
+                    double e;
+                    if (l1 == 0 && l2 == 0) {
+                        e = bp64[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
+                    } else {
+                        const double jB2 = jb64[cf_i4(s8[q], sp1, s8[q + 1], spp)];
+                        double sn = 0.0;
+                        if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
+                        else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
+                        else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
+                        e = cs64[l1 * 31 + l2] + bp64[cf_i2(sp1, s8[q])];
+                        e = e + jB;
+                        e = e + jB2;
+                        e = e + sn;
+                    }
+                    cf_acc<LSE>(best, e + fc);
+                }
+            }
+            const double fm2 = fm2b[i];
+            if (fm2 > CF_HALF64) {
+                double jA = T[CF_O_HC + cf_i2(si, sj1)];
+                if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
+                if (j > 0) jA = jA + T[CF_O_DR + cf_i3(si, sj1, sj)];
+                double v = fm2 + jA;
+                v = v + mp;
+                v = v + mb;
+                cf_acc<LSE>(best, v);
+            }
+            fc_t[db + i] = best;
         }
         __syncthreads();
     }
