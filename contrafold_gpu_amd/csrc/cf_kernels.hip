@@ -1486,35 +1486,54 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
             const int si = s8[i], sj1 = s8[j + 1], si1 = s8[i + 1], sj = s8[j];
             const double jB = jb64[cf_i4(si, sj1, si1, sj)];
             double best = jB + T[CF_O_HP + (s < 30 ? s : 30)];
+            // Candidates in the reference's order (p ascending, q descending),
+            // 8 per batch: the FC loads of a batch are consumed only in its
+            // fold; an empty slot reads cell (0, 0), NEG since the first
+            // diagonal, and is skipped.
             const int pmax = min(i + CF_CMAX, j);
-            for (int p = i; p <= pmax; ++p) {
-                const int l1 = p - i;
-                const int qmin = max(p + 2, p - i + j - CF_CMAX);
-                uint32_t bits = cf_window(pmask + rs8[p + 1] * W, qmin, j);
-                const int sp1 = s8[p + 1], spp = s8[p];
-                while (bits != 0u) {
-                    const int k = 31 - __clz(bits);
-                    bits &= ~(1u << k);
-                    const int q = qmin + k;
-                    const double fc = fc_t[cf_tri(L, p + 1, q - 1)];
-                    if (!(fc > CF_HALF64)) continue;
-                    const int l2 = j - q;
-                    double e;
-                    if (l1 == 0 && l2 == 0) {
-                        e = bp64[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
-                    } else {
-                        const double jB2 = jb64[cf_i4(s8[q], sp1, s8[q + 1], spp)];
-                        double sn = 0.0;
-                        if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
-                        else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
-                        else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
-                        e = cs64[l1 * 31 + l2] + bp64[cf_i2(sp1, s8[q])];
-                        e = e + jB;
-                        e = e + jB2;
-                        e = e + sn;
+            const double e_stack = bp64[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
+            const double sn01 = T[CF_O_B01 + sj], sn10 = T[CF_O_B01 + si1], sn11 = T[CF_O_I11 + cf_i2(si1, sj)];
+            int p = i, qmin = max(i + 2, j - CF_CMAX);
+            uint32_t bits = cf_window(pmask + rs8[p + 1] * W, qmin, j);
+            bool more = true;
+            while (more) {
+                double fv[8], ev[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    while (bits == 0u && p < pmax) {
+                        ++p;
+                        qmin = max(p + 2, p - i + j - CF_CMAX);
+                        bits = cf_window(pmask + rs8[p + 1] * W, qmin, j);
                     }
-                    cf_acc<LSE>(best, e + fc);
+                    int64_t off = 0;
+                    double e = 0.0;
+                    if (bits != 0u) {
+                        const int k = 31 - __clz(bits);
+                        bits &= ~(1u << k);
+                        const int q = qmin + k;
+                        const int l1 = p - i, l2 = j - q;
+                        off = cf_tri(L, p + 1, q - 1);
+                        if (l1 == 0 && l2 == 0) {
+                            e = e_stack;
+                        } else {
+                            const int sp1 = s8[p + 1];
+                            const double jB2 = jb64[cf_i4(s8[q], sp1, s8[q + 1], s8[p])];
+                            const double sn = (l1 == 0 && l2 == 1) ? sn01
+                                            : (l1 == 1 && l2 == 0) ? sn10
+                                            : (l1 == 1 && l2 == 1) ? sn11 : 0.0;
+                            e = cs64[l1 * 31 + l2] + bp64[cf_i2(sp1, s8[q])];
+                            e = e + jB;
+                            e = e + jB2;
+                            e = e + sn;
+                        }
+                    }
+                    fv[u] = fc_t[off];
+                    ev[u] = e;
                 }
+                more = (bits != 0u) || (p < pmax);
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (fv[u] > CF_HALF64) cf_acc<LSE>(best, ev[u] + fv[u]);
             }
             const double fm2 = fm2b[i];
             if (fm2 > CF_HALF64) {
