@@ -1338,10 +1338,10 @@ __host__ __device__ inline CfVitLds cf_vit_plan(int Lmax) {
     o = cf_align(o, 16);
     p.o_st = o; o += (4 * Lmax + 16) * 4;
     o = cf_align(o, 16);
-    p.o_fm2 = o; o += (Lmax + 1) * 8;
+    p.o_fm2 = o; o += 2 * (Lmax + 1) * 8;  // FM2 of the two diagonals of a step
     p.o_pmask = o; o += 5 * p.W * 4;
     o = cf_align(o, 16);
-    p.o_alist = o; o += (Lmax + 1) * 2;
+    p.o_alist = o; o += 2 * (Lmax + 1) * 2;
     o = cf_align(o, 16);
     p.o_cnt = o; o += 16;
     p.o_cs = o; o += 961 * 8;
@@ -1368,9 +1368,9 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
     int8_t* fo8 = (int8_t*)(smem + lp.o_fo8);            // L+3
     int8_t* rs8 = (int8_t*)(smem + lp.o_rs8);            // L+3: pmask row of each base (4 = pairs with nothing)
     uint32_t* st = (uint32_t*)(smem + lp.o_st);          // 4L+16
-    double* fm2b = (double*)(smem + lp.o_fm2);           // FM2 of the current diagonal
+    double* fm2b = (double*)(smem + lp.o_fm2);           // FM2 of diagonals s, s+1: [hi * (Lmax+1) + i]
     uint32_t* pmask = (uint32_t*)(smem + lp.o_pmask);    // 5 x W
-    int16_t* alist = (int16_t*)(smem + lp.o_alist);      // pairable cells of the current diagonal
+    uint16_t* alist = (uint16_t*)(smem + lp.o_alist);    // pairable cells of both diagonals: i | hi << 15
     int* cnt = (int*)(smem + lp.o_cnt);
     double* cs64 = (double*)(smem + lp.o_cs);
     double* jb64 = (double*)(smem + lp.o_jb);
@@ -1410,47 +1410,27 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
     double* __restrict__ fm1_t = FM1 + sd.toff;
     auto allow = [&](int a, int b) -> bool { return cf_canon(s8[a], s8[b]) && !fo8[a] && !fo8[b]; };
 
-    // Per diagonal: (1) the multiloop split FM2, FM1 and FM of every cell (they
-    // read diagonals < s only) and the pairable cells into a list; (2) FC of the
-    // listed cells, the single-branch window walked over the set bits of the
-    // partner masks in the reference's (p ascending, q descending) order.
-    for (int s = 0; s <= L; ++s) {
-        const int64_t db = cf_dbase(L, s);
-        for (int i = tid; i <= L - s; i += NT) {
-            const int j = i + s;
+    // Diagonals two at a time, as in the FP32 inside sweep: FM2 and FC of
+    // diagonal d read FM1 / FM / FC of diagonals <= d-2 only, FM1 and FM read
+    // diagonal d-1.
+    //   A: FM2 of s and s+1 (a lane takes row i of both, sharing FM1[i][.]);
+    //      the pairable cells of both into a list
+    //   B: FC of the listed cells: the single-branch window over the set bits
+    //      of the partner masks, in the reference's (p ascending, q descending)
+    //      order, 8 candidates per batch
+    //   C, D: FM1 and FM of s, then of s+1
+    // Every cell folds its terms in the reference's order.
+    const int FS = Lmax + 1;
+    auto fm_fm1 = [&](int d) {
+        const int64_t db = cf_dbase(L, d);
+        for (int i = tid; i <= L - d; i += NT) {
+            const int j = i + d;
             const bool inner = (i > 0) && (j < L);
-            double fm2 = CF_NEG64;
-            if (inner && s >= 4) {
-                int64_t ia = cf_dbase(L, 2) + i;
-                int64_t ib = cf_dbase(L, s - 2) + i + 2;
-                int t = 2;
-                for (; t + 7 <= s - 2; t += 8) {
-                    double x[8], y[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        x[u] = cf_ld(fm1_t, ia);
-                        y[u] = cf_ld(fm_t, ib);
-                        ia += L + 1 - (t + u);
-                        ib -= L + 1 - (s - (t + u) - 1);
-                        ib += 1;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        if (x[u] > CF_HALF64 && y[u] > CF_HALF64) cf_acc<LSE>(fm2, x[u] + y[u]);
-                }
-                for (; t <= s - 2; ++t) {
-                    const double x = fm1_t[ia], y = fm_t[ib];
-                    if (x > CF_HALF64 && y > CF_HALF64) cf_acc<LSE>(fm2, x + y);
-                    ia += L + 1 - t;
-                    ib -= L + 1 - (s - t - 1);
-                    ib += 1;
-                }
-            }
             double fm1 = CF_NEG64, fmv = CF_NEG64;
-            if (inner && s >= 2) {
+            if (inner && d >= 2) {
                 const int si1 = s8[i + 1], sj = s8[j];
                 if (allow(i + 1, j)) {
-                    const double fcin = fc_t[cf_dbase(L, s - 2) + i + 1];
+                    const double fcin = fc_t[cf_dbase(L, d - 2) + i + 1];
                     if (fcin > CF_HALF64) {
                         double jA = T[CF_O_HC + cf_i2(sj, si1)];
                         if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, s8[j + 1])];
@@ -1461,35 +1441,94 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                         cf_acc<LSE>(fm1, v);
                     }
                 }
-                const double f1n = fm1_t[cf_dbase(L, s - 1) + i + 1];
+                const double fm2 = fm2b[(d & 1) * FS + i];
+                const double f1n = fm1_t[cf_dbase(L, d - 1) + i + 1];
                 if (f1n > CF_HALF64) cf_acc<LSE>(fm1, f1n + mu);
                 if (fm2 > CF_HALF64) cf_acc<LSE>(fmv, fm2);
-                const double fml = fm_t[cf_dbase(L, s - 1) + i];
+                const double fml = fm_t[cf_dbase(L, d - 1) + i];
                 if (fml > CF_HALF64) cf_acc<LSE>(fmv, fml + mu);
                 if (fm1 > CF_HALF64) cf_acc<LSE>(fmv, fm1);
             }
             fm1_t[db + i] = fm1;
             fm_t[db + i] = fmv;
-            if (inner && allow(i, j + 1)) {
-                fm2b[i] = fm2;
-                alist[atomicAdd(&cnt[s & 1], 1)] = (int16_t)i;
-            } else {
-                fc_t[db + i] = CF_NEG64;
+        }
+    };
+    for (int s = 0; s <= L; s += 2) {
+        const int step = s >> 1;
+        const int n0 = L + 1 - s;
+        const int n1 = (s + 1 <= L) ? L - s : 0;
+        // ---- A
+        for (int i = tid; i < n0; i += NT) {
+            const int j0 = i + s;
+            const bool inner0 = (i > 0) && (j0 < L);
+            const bool inner1 = (i < n1) && (i > 0) && (j0 + 1 < L);
+            double a0 = CF_NEG64, a1 = CF_NEG64;
+            if (inner0 && s >= 3) {
+                // t = 2..s-2 for both cells (FM1[i][i+t] shared), t = s-1 for (i, i+s+1)
+                int64_t ia = cf_dbase(L, 2) + i;
+                int64_t ib0 = cf_dbase(L, s - 2) + i + 2;
+                int64_t ib1 = cf_dbase(L, s - 1) + i + 2;
+                int t = 2;
+                constexpr int VB = 4;  // FP64 terms per batch (3 x 4 doubles in flight)
+                for (; t + VB - 1 <= s - 2; t += VB) {
+                    double x[VB], y0[VB], y1[VB];
+#pragma unroll
+                    for (int u = 0; u < VB; ++u) {
+                        x[u] = cf_ld(fm1_t, ia);
+                        y0[u] = cf_ld(fm_t, ib0);
+                        y1[u] = cf_ld(fm_t, ib1);
+                        ia += L + 1 - (t + u);
+                        ib0 -= L - (s - (t + u) - 1);
+                        ib1 -= L - (s - (t + u));
+                    }
+#pragma unroll
+                    for (int u = 0; u < VB; ++u) {
+                        if (x[u] > CF_HALF64 && y0[u] > CF_HALF64) cf_acc<LSE>(a0, x[u] + y0[u]);
+                        if (x[u] > CF_HALF64 && y1[u] > CF_HALF64) cf_acc<LSE>(a1, x[u] + y1[u]);
+                    }
+                }
+                for (; t <= s - 2; ++t) {
+                    const double x = cf_ld(fm1_t, ia), y0 = cf_ld(fm_t, ib0), y1 = cf_ld(fm_t, ib1);
+                    if (x > CF_HALF64 && y0 > CF_HALF64) cf_acc<LSE>(a0, x + y0);
+                    if (x > CF_HALF64 && y1 > CF_HALF64) cf_acc<LSE>(a1, x + y1);
+                    ia += L + 1 - t;
+                    ib0 -= L - (s - t - 1);
+                    ib1 -= L - (s - t);
+                }
+                if (inner1) {
+                    const double x = cf_ld(fm1_t, ia), y1 = cf_ld(fm_t, ib1);
+                    if (x > CF_HALF64 && y1 > CF_HALF64) cf_acc<LSE>(a1, x + y1);
+                }
+            }
+#pragma unroll
+            for (int hi = 0; hi < 2; ++hi) {
+                if (hi == 1 && i >= n1) break;
+                const int d = s + hi;
+                const bool inner = hi ? inner1 : inner0;
+                fm2b[(d & 1) * FS + i] = inner ? (hi ? a1 : a0) : CF_NEG64;
+                if (inner && allow(i, i + d + 1)) {
+                    alist[atomicAdd(&cnt[step & 1], 1)] = (uint16_t)(i | (hi << 15));
+                } else {
+                    fc_t[cf_dbase(L, d) + i] = CF_NEG64;
+                }
             }
         }
         __syncthreads();
-        const int nal = cnt[s & 1];
-        if (tid == 0) cnt[(s + 1) & 1] = 0;
+        // ---- B
+        const int nal = cnt[step & 1];
+        if (tid == 0) cnt[(step + 1) & 1] = 0;
         for (int idx = tid; idx < nal; idx += NT) {
-            const int i = alist[idx];
-            const int j = i + s;
+            const int e0 = alist[idx];
+            const int i = e0 & 0x7fff;
+            const int d = s + (e0 >> 15);
+            const int j = i + d;
             const int si = s8[i], sj1 = s8[j + 1], si1 = s8[i + 1], sj = s8[j];
             const double jB = jb64[cf_i4(si, sj1, si1, sj)];
-            double best = jB + T[CF_O_HP + (s < 30 ? s : 30)];
+            double best = jB + T[CF_O_HP + (d < 30 ? d : 30)];
             // Candidates in the reference's order (p ascending, q descending),
             // 8 per batch: the FC loads of a batch are consumed only in its
             // fold; an empty slot reads cell (0, 0), NEG since the first
-            // diagonal, and is skipped.
+            // step, and is skipped.
             const int pmax = min(i + CF_CMAX, j);
             const double e_stack = bp64[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
             const double sn01 = T[CF_O_B01 + sj], sn10 = T[CF_O_B01 + si1], sn11 = T[CF_O_I11 + cf_i2(si1, sj)];
@@ -1535,7 +1574,7 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                 for (int u = 0; u < 8; ++u)
                     if (fv[u] > CF_HALF64) cf_acc<LSE>(best, ev[u] + fv[u]);
             }
-            const double fm2 = fm2b[i];
+            const double fm2 = fm2b[(d & 1) * FS + i];
             if (fm2 > CF_HALF64) {
                 double jA = T[CF_O_HC + cf_i2(si, sj1)];
                 if (i < L) jA = jA + T[CF_O_DL + cf_i3(si, sj1, si1)];
@@ -1545,9 +1584,16 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                 v = v + mb;
                 cf_acc<LSE>(best, v);
             }
-            fc_t[db + i] = best;
+            fc_t[cf_dbase(L, d) + i] = best;
         }
         __syncthreads();
+        // ---- C, D
+        fm_fm1(s);
+        __syncthreads();
+        if (s + 1 <= L) {
+            fm_fm1(s + 1);
+            __syncthreads();
+        }
     }
     if (LSE) {
         // F5 chain (cpu.py:259-275), one lane, reference order
