@@ -158,6 +158,19 @@ class Server:
         return lat / 1e3, wall.value
 
 
+def steady_rate(arrival_s, latency_ms):
+    """Completions per second while the middle 80% of the arrivals come in
+    (requests sorted by arrival; a negative latency marks a failed request).
+    Unlike n / wall, this does not count the drain of the last long folds
+    after the final arrival, so a server that keeps up reads ~ the offered rate."""
+    arr = np.asarray(arrival_s, np.float64)
+    lat = np.asarray(latency_ms, np.float64)
+    n = arr.size
+    done = arr + np.where(lat >= 0, lat, np.inf) / 1e3
+    lo_t, hi_t = arr[n // 10], arr[(9 * n) // 10]
+    return float(((done >= lo_t) & (done < hi_t)).sum()) / max(hi_t - lo_t, 1e-9)
+
+
 def plain_throughput(engine, trace, repeats=2):
     """Folds/s of the same request mix folded as plain batches (one batch per
     kind, inputs resident, steady state)."""
@@ -205,13 +218,7 @@ def benchmark(engine, P, n=12000, fracs=(0.25, 0.5, 0.7, 0.85, 1.0), seed=5, log
             tr = workloads.serving_trace(n, qps, seed=seed)
             lat, wall, wait = srv.replay(tr, with_wait=True)
             ok = lat >= 0
-            # steady state: completions while the middle 80% of the arrivals
-            # come in, per second (the whole-trace figure also counts the
-            # drain of the last long folds after the final arrival)
-            arr = np.asarray(tr["arrival_s"], np.float64)
-            done = arr + np.where(ok, lat, np.inf) / 1e3
-            lo_t, hi_t = arr[n // 10], arr[(9 * n) // 10]
-            sustained = float(((done >= lo_t) & (done < hi_t)).sum()) / max(hi_t - lo_t, 1e-9)
+            sustained = steady_rate(tr["arrival_s"], lat)
             row = {"offered_frac": f, "offered_qps": round(qps, 1), "served_folds_per_s": round(n / wall, 1),
                    "served_vs_plain": round(n / wall / plain, 4),
                    "sustained_folds_per_s": round(sustained, 1), "sustained_vs_plain": round(sustained / plain, 4),

@@ -165,3 +165,61 @@ def test_work_counts_closed_forms():
     assert c["posterior"] == 0 and c["exterior"] == 100
     c = workcount.counts("GGGGAAAACCCC")
     assert c["cells"] == 13 * 14 // 2 and c["inside"] > 0
+
+
+def test_serving_steady_rate():
+    """The steady-state served rate: a server that keeps up reads the offered
+    rate whatever its latency; one that falls behind reads its own capacity;
+    failed requests never count."""
+    from contrafold_gpu_amd.serving import steady_rate
+    n, qps = 10000, 1000.0
+    arr = np.arange(n) / qps
+    assert abs(steady_rate(arr, np.full(n, 50.0)) - qps) / qps < 0.01      # constant 50 ms latency
+    assert abs(steady_rate(arr, np.full(n, 900.0)) - qps) / qps < 0.01     # long but steady
+    # capacity 500/s: request k completes at k / 500 s
+    lat = (np.arange(n) / 500.0 - arr) * 1e3
+    assert abs(steady_rate(arr, lat) - 500.0) / 500.0 < 0.02
+    failed = np.full(n, 50.0)
+    failed[::2] = -1.0
+    assert abs(steady_rate(arr, failed) - qps / 2) / qps < 0.01
+
+
+def test_autotune_with_a_fake_engine(tmp_path, monkeypatch):
+    """Host logic of the tuner: it times every candidate on at least a third of
+    the workload (up to 8,000 sequences), installs the fastest and caches it."""
+    from contrafold_gpu_amd import autotune
+    monkeypatch.setenv("CF_TUNE_CACHE", str(tmp_path / "tune.json"))
+
+    class FakeBatch:
+        def __init__(self, eng, seqs):
+            self.eng, self.n = eng, len(seqs)
+            eng.sizes.append(self.n)
+        def __enter__(self):
+            return self
+        def __exit__(self, *a):
+            return False
+        def run(self, flags, **kw):
+            pass
+        def last_ms(self):  # the candidate with a 256-wide last bucket is fastest
+            return 5.0 if self.eng.cur[1][-1] == 256 else 9.0
+
+    class FakeEngine:
+        def __init__(self):
+            self.sizes, self.cur, self.installed = [], None, []
+        def set_tuning(self, edges, threads):
+            self.cur = (list(edges), list(threads))
+            self.installed.append(self.cur)
+        def batch(self, seqs, masks=None):
+            return FakeBatch(self, seqs)
+
+    eng = FakeEngine()
+    seqs = ["ACGU" * 10] * 9000
+    res = autotune.tune(eng, seqs, sample=1500, repeats=1)
+    assert not res["cached"] and len(res["tried"]) == len(autotune.CANDIDATES)
+    assert set(eng.sizes) == {3000}                      # a third of 9,000, not 1,500
+    assert res["threads"][-1] == 256 and eng.cur == (res["edges"], res["threads"])
+    again = autotune.tune(eng, seqs, sample=1500, repeats=1)
+    assert again["cached"] and again["threads"] == res["threads"]
+    fe = FakeEngine()
+    big = autotune.tune(fe, ["ACGU" * 30] * 60000, sample=1500, repeats=1, use_cache=False)
+    assert big["ms"] == 5.0 and set(fe.sizes) == {8000}  # capped at 8,000
