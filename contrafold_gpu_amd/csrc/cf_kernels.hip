@@ -1177,7 +1177,7 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
 // exact, so the strict '>' scan over the candidate list equals a parallel
 // argmax with lowest-index ties; here each lane still scans its own cell in
 // order, which is both exact and simple.  Output: dot-bracket bytes.
-__global__ void __launch_bounds__(256) cf_mea_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
+__global__ void __launch_bounds__(512) cf_mea_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
                                                      int nlaunch, const uint8_t* __restrict__ codes,
                                                      const uint8_t* __restrict__ masks, const float* __restrict__ POSTF,
                                                      const float* __restrict__ gammas, float* __restrict__ SC,
