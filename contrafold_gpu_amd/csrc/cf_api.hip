@@ -191,12 +191,12 @@ int launch_post(cf_batch* b, const Bucket& bk, hipStream_t st) {
     cf_ctx* c = b->ctx;
     const int nl = bk.end - bk.begin;
     if (nl <= 0) return CF_OK;
-    // The kPack waves of a workgroup split one sequence's diagonals: the
-    // kernel waits on global FCo reads, and four times the waves of the
-    // packed form hide that better (12 vs 22 ms for 3000 x 200 nt).
-    // A wide bucket's workgroup width (up to 16 waves) splits long folds
-    // further, as far as the per-wave LDS areas fit.
-    int split = std::max(kPack, bk.threads / 64);
+    // The waves of a workgroup split one sequence's diagonals: the kernel
+    // waits on global FCo reads, which more waves per sequence hide better.
+    // Eight per sequence: 27.9 ms at 10,000 x 200 nt, against 30.3 for four
+    // and 41 for two.  A wide bucket's workgroup width (up to 16 waves) splits
+    // long folds further, as far as the per-wave LDS areas fit.
+    int split = std::max(8, bk.threads / 64);
     CfLds plan = cf_lds_plan(bk.Lmax, false, 4, split);
     while (plan.total > c->lds_max && split > kPack) plan = cf_lds_plan(bk.Lmax, false, 4, --split);
     if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the posterior kernel's LDS plan");
