@@ -243,11 +243,26 @@ int launch_viterbi(cf_batch* b, const Bucket& bk, bool lse, hipStream_t st) {
 
 // ---- whole-batch launches
 
+// The sampler's per-wave LDS slice holds the F5 row, codes and mask as well
+// as the stack top when four 512-thread workgroups still fit a CU with it.
+bool samp_local(int Lmax, bool masks) {
+    const size_t fixed = sizeof(CfParams<float>) + 2 * 496;
+    return 4 * (fixed + (size_t)CF_SAMP_WAVES * cf_samp_pitch(Lmax, true, masks)) <= 160 * 1024;
+}
+
 int launch_sample(cf_batch* b, uint64_t seed, hipStream_t st) {
     cf_ctx* c = b->ctx;
-    hipLaunchKernelGGL(cf_sample_kernel, dim3(b->sgrid), dim3(256), 0, st, b->d_seqs, b->d_order, b->n, b->n_samples,
-                       seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2, b->F5, b->stack,
-                       b->stack_cap, b->samp_out, b->d_soff);
+    const bool masks = b->d_masks != nullptr;
+    const bool local = samp_local(b->Lmax, masks);
+    const size_t lds = (size_t)CF_SAMP_WAVES * cf_samp_pitch(b->Lmax, local, masks);
+    if (local)
+        hipLaunchKernelGGL(cf_sample_kernel<true>, dim3(b->sgrid), dim3(CF_SAMP_WAVES * WAVE), lds, st, b->d_seqs,
+                           b->d_order, b->n, b->n_samples, seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC,
+                           b->FM, b->FM1, b->FM2, b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff, b->Lmax);
+    else
+        hipLaunchKernelGGL(cf_sample_kernel<false>, dim3(b->sgrid), dim3(CF_SAMP_WAVES * WAVE), lds, st, b->d_seqs,
+                           b->d_order, b->n, b->n_samples, seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC,
+                           b->FM, b->FM1, b->FM2, b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff, b->Lmax);
     CF_HIP(c, hipGetLastError(), "sample launch");
     return CF_OK;
 }
@@ -318,9 +333,10 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
     }
     if (flags & CF_SAMPLE) {
         const int64_t tasks = (int64_t)b->n * n_samples;
-        // one wave per (sequence, sample), four waves per workgroup, a persistent
-        // grid of up to 8 workgroups per CU; each wave owns a traceback stack
-        const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>((tasks + 3) / 4, (int64_t)c->n_cu * 8));
+        // one wave per (sequence, sample), eight waves per workgroup, a persistent
+        // grid of up to 4 workgroups per CU; each wave owns a traceback stack
+        const int sgrid = (int)std::max<int64_t>(
+            1, std::min<int64_t>((tasks + CF_SAMP_WAVES - 1) / CF_SAMP_WAVES, (int64_t)c->n_cu * 4));
         const int cap = 2 * b->Lmax + 8;
         if (n_samples > b->smp_cap || sgrid > b->sgrid || cap > b->stack_cap) {
             region_put(c, b->r_smp);
@@ -328,12 +344,12 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
             const int g_cap = std::max(sgrid, b->sgrid);
             auto lay = [&](Carver& k) {
                 k.take<uint8_t>((int64_t)ns_cap * b->total_chars);
-                k.take<uint32_t>((int64_t)g_cap * 4 * cap);
+                k.take<uint32_t>((int64_t)g_cap * CF_SAMP_WAVES * cap);
             };
             if ((rc = region_get(c, b->r_smp, measure(lay)))) return rc;
             Carver k{(char*)b->r_smp.p};
             b->samp_out = k.take<uint8_t>((int64_t)ns_cap * b->total_chars);
-            b->stack = k.take<uint32_t>((int64_t)g_cap * 4 * cap);
+            b->stack = k.take<uint32_t>((int64_t)g_cap * CF_SAMP_WAVES * cap);
             b->smp_cap = ns_cap;
             b->stack_cap = cap;
         }

@@ -1841,6 +1841,24 @@ struct CfRng {
     }
 };
 
+// Inclusive prefix sum over the wave with DPP moves (no LDS round trips):
+// shifts by 1, 2, 4 and 8 within each 16-lane row, then lane 15's row total
+// is added to rows 1 and 3 and lane 31's running total to rows 2 and 3.  A
+// lane a move does not write keeps the old value, zero.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float cf_dpp_take(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, kRowMask, 0xf, false));
+}
+__device__ __forceinline__ float cf_wave_incl_sum(float v) {
+    v = v + cf_dpp_take<0x111, 0xf>(v);  // row_shr:1
+    v = v + cf_dpp_take<0x112, 0xf>(v);  // row_shr:2
+    v = v + cf_dpp_take<0x114, 0xf>(v);  // row_shr:4
+    v = v + cf_dpp_take<0x118, 0xf>(v);  // row_shr:8
+    v = v + cf_dpp_take<0x142, 0xa>(v);  // row_bcast:15
+    v = v + cf_dpp_take<0x143, 0xc>(v);  // row_bcast:31
+    return v;
+}
+
 // Stochastic traceback, one wave per (sequence, sample): the reference's state
 // machine (gpu.py:220-477) with its candidate order, walked cooperatively.
 // Each state draws one uniform r (Philox4x32-10, keyed by the sequence's key
@@ -1852,37 +1870,47 @@ struct CfRng {
 // with a second draw.  Sums are formed in a different order than the
 // reference's serial loop and the hardware exp2 is used, so sampling is
 // checked statistically, not bit for bit.
-__device__ __forceinline__ float cf_wave_incl_sum(float v, int lane) {
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        const float t = __shfl_up(v, d, WAVE);
-        if (lane >= d) v = v + t;
-    }
-    return v;
-}
-
 // One chunk of a candidate scan.  `c` is the running sum before the chunk.
 // Returns the lane of the picked candidate, or -1 (then `c` has the chunk's
 // sum added and `last` the highest lane holding a candidate, if any).
 __device__ __forceinline__ int cf_scan_pick(float p, bool valid, float r, float& c, int& last, int lane) {
-    const float cum = cf_wave_incl_sum(valid ? p : 0.0f, lane);
+    const float cum = cf_wave_incl_sum(valid ? p : 0.0f);
     const float run = c + cum;
     const uint64_t hit = __ballot(valid && r <= run);
     if (hit) return __ffsll((unsigned long long)hit) - 1;
-    c = c + __shfl(cum, WAVE - 1, WAVE);
+    c = c + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cum), WAVE - 1));
     const uint64_t any = __ballot(valid);
     if (any) last = 63 - __clzll((long long)any);
     return -1;
 }
 
-// Eight waves per SIMD (the register cap spills ~112 bytes per lane of cold
-// state) beat five unspilled ones: 31 vs 41 ms for 10,000 x 200 nt x 100.
-__global__ void __launch_bounds__(256, 8) cf_sample_kernel(
+// Per-wave LDS slice of the sampler: the first CF_SAMP_STK entries of the
+// traceback stack (deeper entries go to the wave's HBM stack) and, in the
+// local form, the sequence's F5 row, base codes and mask.  A traceback is a
+// chain of a few hundred dependent reads; with the stack, the exterior row
+// and every pairing test in LDS, only the table cells a state scores come
+// from L2/HBM, and the next candidate chunk's cells are requested before
+// the current chunk is scanned.  The local form is used when the slices
+// still let four 512-thread workgroups share a CU.
+#define CF_SAMP_STK 128
+#define CF_SAMP_WAVES 8
+__host__ __device__ inline int cf_samp_row(int lmax) { return (lmax + 2 + 3) & ~3; }
+__host__ __device__ inline int cf_samp_pitch(int lmax, bool local, bool masks) {
+    const int n = cf_samp_row(lmax);
+    return CF_SAMP_STK * 4 + (local ? 5 * n + (masks ? n : 0) : 0);
+}
+
+// Eight waves per SIMD (the register cap spills a little cold state) beat
+// five unspilled ones: 31 vs 41 ms for 10,000 x 200 nt x 100.
+template <bool kLocal>
+__global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ sidx, int nseq, int nsamp, uint64_t seed,
     const uint64_t* __restrict__ keys, const uint8_t* __restrict__ codes, const uint8_t* __restrict__ masks,
     const CfParams<float>* __restrict__ P, const float* __restrict__ FC, const float* __restrict__ FM,
     const float* __restrict__ FM1, const float* __restrict__ FM2, const float* __restrict__ F5,
-    uint32_t* __restrict__ stack, int stack_cap, uint8_t* __restrict__ out, const int64_t* __restrict__ out_off) {
+    uint32_t* __restrict__ stack, int stack_cap, uint8_t* __restrict__ out, const int64_t* __restrict__ out_off,
+    int lmax) {
+    extern __shared__ __align__(16) unsigned char smem[];
     __shared__ CfParams<float> sp;
     __shared__ uint8_t wl1[496], wl2[496];  // single-branch window order: l1 ascending, l2 ascending
     for (int t = threadIdx.x; t < CF_NTAB; t += blockDim.x) sp.t[t] = P->t[t];
@@ -1904,17 +1932,44 @@ __global__ void __launch_bounds__(256, 8) cf_sample_kernel(
     const int64_t gw = (int64_t)blockIdx.x * nw + wv;
     const int64_t gws = (int64_t)gridDim.x * nw;
     uint32_t* __restrict__ st = stack + gw * stack_cap;
+    const bool has_mask = masks != nullptr;
+    unsigned char* slice = smem + (size_t)wv * cf_samp_pitch(lmax, kLocal, has_mask);
+    uint32_t* lstk = (uint32_t*)slice;
+    float* lf5 = (float*)(slice + CF_SAMP_STK * 4);
+    uint8_t* lcod = slice + CF_SAMP_STK * 4 + 4 * cf_samp_row(lmax);
+    uint8_t* lmsk = lcod + cf_samp_row(lmax);
     const float* __restrict__ T = sp.t;
     const float* __restrict__ JBt = sp.JB;
-    const float mb = T[CF_O_MB], mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
+    const float mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
+    int prev_b = -1;
     for (int64_t task = gw; task < total; task += gws) {
         const int li = (int)(task / nsamp);
         const int smp = (int)(task % nsamp);
         const int b = sidx[li];
         const CfSeq sd = seqs[b];
         const int L = sd.L;
-        const uint8_t* __restrict__ s = codes + sd.coff;
-        const uint8_t* __restrict__ fo = masks ? masks + sd.coff : nullptr;
+        const uint8_t* __restrict__ s;
+        const uint8_t* __restrict__ fo;
+        const float* __restrict__ f5;
+        if constexpr (kLocal) {
+            if (b != prev_b) {
+                __builtin_amdgcn_wave_barrier();  // the previous sequence's reads precede the refill
+                for (int q = lane; q < L + 2; q += WAVE) {
+                    lf5[q] = F5[sd.coff + q];
+                    lcod[q] = codes[sd.coff + q];
+                    if (has_mask) lmsk[q] = masks[sd.coff + q];
+                }
+                __builtin_amdgcn_wave_barrier();
+                prev_b = b;
+            }
+            s = lcod;
+            fo = has_mask ? lmsk : nullptr;
+            f5 = lf5;
+        } else {
+            s = codes + sd.coff;
+            fo = has_mask ? masks + sd.coff : nullptr;
+            f5 = F5 + sd.coff;
+        }
         auto allow = [&](int a, int c) -> bool {
             return cf_canon(s[a], s[c]) && (fo == nullptr || (fo[a] == 0 && fo[c] == 0));
         };
@@ -1922,7 +1977,6 @@ __global__ void __launch_bounds__(256, 8) cf_sample_kernel(
         const float* __restrict__ fm_t = FM + sd.toff;
         const float* __restrict__ fm1_t = FM1 + sd.toff;
         const float* __restrict__ fm2_t = FM2 + sd.toff;
-        const float* __restrict__ f5 = F5 + sd.coff;
         uint8_t* __restrict__ o = out + out_off[b] + (int64_t)smp * L;
         for (int q = lane; q < L; q += WAVE) o[q] = '.';
         CfRng rng;
@@ -1932,25 +1986,35 @@ __global__ void __launch_bounds__(256, 8) cf_sample_kernel(
         rng.sample = (uint32_t)smp;
         rng.ctr = 0;
         rng.have = 0;
-        // the stack is wave-uniform: every lane stores the same entry and reads back its own store
+        // the stack is wave-uniform: every lane stores the same entry, and a
+        // popped entry is read into scalar registers
         int sp_ = 0;
         auto push = [&](int typ, int i, int j) {
-            if (sp_ < stack_cap) st[sp_++] = (uint32_t)typ | ((uint32_t)i << 2) | ((uint32_t)j << 17);
+            const uint32_t e = (uint32_t)typ | ((uint32_t)i << 2) | ((uint32_t)j << 17);
+            if (sp_ < CF_SAMP_STK) lstk[sp_++] = e;
+            else if (sp_ < stack_cap) st[sp_++] = e;
         };
-        // multiloop split of FM2[i][j]: k with weight FM1[i][k] + FM[k][j]
+        // multiloop split of FM2[i][j]: k with weight FM1[i][k] + FM[k][j];
+        // the next chunk's cells are requested before the current one is scanned
         auto split = [&](int i, int j, float fm2) -> int {
             const float r2 = rng.next();
             float c2 = 0.0f;
             int last = -1;
-            for (int k0 = i + 1; k0 < j; k0 += WAVE) {
-                const int k = k0 + lane;
-                float p = 0.0f;
-                bool valid = false;
+            auto cells = [&](int k, float& x, float& y) {
+                x = CF_NEG;
+                y = CF_NEG;
                 if (k < j) {
-                    const float x = fm1_t[cf_tri32(L, i, k)], y = fm_t[cf_tri32(L, k, j)];
-                    valid = x > CF_HALF && y > CF_HALF;
-                    p = __expf(x + y - fm2);
+                    x = fm1_t[cf_tri32(L, i, k)];
+                    y = fm_t[cf_tri32(L, k, j)];
                 }
+            };
+            float xn, yn;
+            cells(i + 1 + lane, xn, yn);
+            for (int k0 = i + 1; k0 < j; k0 += WAVE) {
+                const float x = xn, y = yn;
+                if (k0 + WAVE < j) cells(k0 + WAVE + lane, xn, yn);
+                const bool valid = x > CF_HALF && y > CF_HALF;
+                const float p = valid ? __expf(x + y - fm2) : 0.0f;
                 int lastc = -1;
                 const int t = cf_scan_pick(p, valid, r2, c2, lastc, lane);
                 if (t >= 0) return k0 + t;
@@ -1961,7 +2025,10 @@ __global__ void __launch_bounds__(256, 8) cf_sample_kernel(
         push(0, 0, L);
         while (sp_ > 0) {
             --sp_;
-            const uint32_t e0 = st[sp_];
+            uint32_t e0;
+            if (sp_ < CF_SAMP_STK) e0 = lstk[sp_];
+            else e0 = st[sp_];
+            e0 = __builtin_amdgcn_readfirstlane(e0);
             const int typ = e0 & 3, i = (e0 >> 2) & 0x7fff, j = (e0 >> 17) & 0x7fff;
             if (typ == 0) {  // F5(j): j unpaired, or pair (k+1, j) closing an exterior helix
                 if (j == 0) continue;
@@ -1972,21 +2039,25 @@ __global__ void __launch_bounds__(256, 8) cf_sample_kernel(
                 if (!(r <= c)) {
                     const int sj = s[j];
                     int last = -1;
+                    // FC cell of the pair (k+1, j), NEG when k+1 cannot pair with j
+                    auto cell = [&](int k) -> float {
+                        return (k < j && allow(k + 1, j)) ? fc_t[cf_tri32(L, k + 1, j - 1)] : CF_NEG;
+                    };
+                    float fcn = cell(lane);
                     for (int k0 = 0; k0 < j; k0 += WAVE) {
                         const int k = k0 + lane;
+                        const float fc = fcn;
+                        if (k0 + WAVE < j) fcn = cell(k + WAVE);
                         float p = 0.0f;
                         bool valid = false;
-                        if (k < j && allow(k + 1, j)) {
-                            const float fc = fc_t[cf_tri32(L, k + 1, j - 1)];
-                            if (fc > CF_HALF && f5[k] > CF_HALF) {
-                                const int sk1 = s[k + 1];
-                                float jA = T[CF_O_HC + cf_i2(sj, sk1)];
-                                if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s[j + 1])];
-                                if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s[k])];
-                                const float w = f5[k] + fc + ep + T[CF_O_BP + cf_i2(sk1, sj)] + jA;
-                                p = __expf(w - tot);
-                                valid = true;
-                            }
+                        if (fc > CF_HALF && f5[k] > CF_HALF) {
+                            const int sk1 = s[k + 1];
+                            float jA = T[CF_O_HC + cf_i2(sj, sk1)];
+                            if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, sk1, s[j + 1])];
+                            if (k > 0) jA = jA + T[CF_O_DR + cf_i3(sj, sk1, s[k])];
+                            const float w = f5[k] + fc + ep + T[CF_O_BP + cf_i2(sk1, sj)] + jA;
+                            p = __expf(w - tot);
+                            valid = true;
                         }
                         int lastc = -1;
                         const int t = cf_scan_pick(p, valid, r, c, lastc, lane);
@@ -2007,42 +2078,48 @@ __global__ void __launch_bounds__(256, 8) cf_sample_kernel(
                     push(0, 0, pick);
                 }
             } else if (typ == 1) {  // FC(i, j): pair (i, j+1) already recorded
+                // single-branch window: rows l1 = 0 .. min(30, d-2), q = j - l2 >= p + 2
+                const int d = j - i;
+                const int nrows = min(CF_CMAX, d - 2) + 1;
+                const int tend = nrows > 0 ? 31 * nrows - nrows * (nrows - 1) / 2 : 0;
+                // FC cell of candidate t's inner pair (i+l1+1, j-l2), NEG when absent
+                auto cell = [&](int t) -> float {
+                    if (t >= tend) return CF_NEG;
+                    const int pp = i + wl1[t], q = j - wl2[t];
+                    return (q >= pp + 2 && allow(pp + 1, q)) ? fc_t[cf_tri32(L, pp + 1, q - 1)] : CF_NEG;
+                };
+                // the first chunk's cells and the multiloop total are requested with the state's own
+                float fcn = cell(lane);
                 const float tot = fc_t[cf_tri32(L, i, j)];
+                const float fm2 = fm2_t[cf_tri32(L, i, j)];
                 const int si = s[i], sj1 = s[j + 1], si1 = s[i + 1], sj = s[j];
                 const float jB = JBt[cf_i4(si, sj1, si1, sj)];
-                const int d = j - i;
                 const float r = rng.next();
                 float c = __expf(jB + T[CF_O_HP + (d <= 30 ? d : 30)] - tot);
                 if (r <= c) continue;  // hairpin
-                // single-branch window: rows l1 = 0 .. min(30, d-2), q = j - l2 >= p + 2
-                const int nrows = min(CF_CMAX, d - 2) + 1;
-                const int tend = nrows > 0 ? 31 * nrows - nrows * (nrows - 1) / 2 : 0;
                 int pick = -1, last = -1;
                 for (int t0 = 0; t0 < tend; t0 += WAVE) {
                     const int t = t0 + lane;
+                    const float fc = fcn;
+                    if (t0 + WAVE < tend) fcn = cell(t + WAVE);
                     float p = 0.0f;
                     bool valid = false;
-                    if (t < tend) {
+                    if (fc > CF_HALF) {
                         const int l1 = wl1[t], l2 = wl2[t];
                         const int pp = i + l1, q = j - l2;
-                        if (q >= pp + 2 && allow(pp + 1, q)) {
-                            const float fc = fc_t[cf_tri32(L, pp + 1, q - 1)];
-                            if (fc > CF_HALF) {
-                                float e;
-                                if (l1 == 0 && l2 == 0) {
-                                    e = T[CF_O_BP + cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
-                                } else {
-                                    float sn = 0.0f;
-                                    if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
-                                    else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
-                                    else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
-                                    e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s[pp + 1], s[q])] + jB +
-                                        JBt[cf_i4(s[q], s[pp + 1], s[q + 1], s[pp])] + sn;
-                                }
-                                p = __expf(e + fc - tot);
-                                valid = true;
-                            }
+                        float e;
+                        if (l1 == 0 && l2 == 0) {
+                            e = T[CF_O_BP + cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
+                        } else {
+                            float sn = 0.0f;
+                            if (l1 == 0 && l2 == 1) sn = T[CF_O_B01 + sj];
+                            else if (l1 == 1 && l2 == 0) sn = T[CF_O_B01 + si1];
+                            else if (l1 == 1 && l2 == 1) sn = T[CF_O_I11 + cf_i2(si1, sj)];
+                            e = T[CF_O_CS + l1 * 31 + l2] + T[CF_O_BP + cf_i2(s[pp + 1], s[q])] + jB +
+                                JBt[cf_i4(s[q], s[pp + 1], s[q + 1], s[pp])] + sn;
                         }
+                        p = __expf(e + fc - tot);
+                        valid = true;
                     }
                     int lastc = -1;
                     const int tp = cf_scan_pick(p, valid, r, c, lastc, lane);
@@ -2053,7 +2130,6 @@ __global__ void __launch_bounds__(256, 8) cf_sample_kernel(
                     if (lastc >= 0) last = t0 + lastc;
                 }
                 if (pick < 0) {
-                    const float fm2 = fm2_t[cf_tri32(L, i, j)];
                     if (fm2 > CF_HALF) {
                         // multiloop: taken when r passed every single-branch candidate
                         const int kk = split(i, j, fm2);
@@ -2091,28 +2167,26 @@ __global__ void __launch_bounds__(256, 8) cf_sample_kernel(
                 } else if (opt == 2) {
                     push(3, i, j);
                 }
-            } else {  // FM1(i, j): pair (i+1, j) or i+1 unpaired
+            } else {  // FM1(i, j), i < j: pair (i+1, j) or i+1 unpaired
+                const bool pc = allow(i + 1, j);
                 const float tot = fm1_t[cf_tri32(L, i, j)];
+                const float fc = pc ? fc_t[cf_tri32(L, i + 1, j - 1)] : CF_NEG;
+                const float wun = fm1_t[cf_tri32(L, i + 1, j)];
                 const float r = rng.next();
-                float c = 0.0f;
                 bool paired = false;
-                if (allow(i + 1, j)) {
-                    const float fc = fc_t[cf_tri32(L, i + 1, j - 1)];
-                    if (fc > CF_HALF) {
-                        const int si1 = s[i + 1], sj = s[j];
-                        float jA = T[CF_O_HC + cf_i2(sj, si1)];
-                        if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, s[j + 1])];
-                        if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, s[i])];
-                        c += __expf(fc + jA + mp + T[CF_O_BP + cf_i2(si1, sj)] - tot);
-                        const float wun = fm1_t[cf_tri32(L, i + 1, j)];
-                        paired = (r <= c) || !(wun > CF_HALF);
-                    }
+                if (fc > CF_HALF) {
+                    const int si1 = s[i + 1], sj = s[j];
+                    float jA = T[CF_O_HC + cf_i2(sj, si1)];
+                    if (j < L) jA = jA + T[CF_O_DL + cf_i3(sj, si1, s[j + 1])];
+                    if (i > 0) jA = jA + T[CF_O_DR + cf_i3(sj, si1, s[i])];
+                    const float c = __expf(fc + jA + mp + T[CF_O_BP + cf_i2(si1, sj)] - tot);
+                    paired = (r <= c) || !(wun > CF_HALF);
                 }
                 if (paired) {
                     o[i] = '(';
                     o[j - 1] = ')';
                     push(1, i + 1, j - 1);
-                } else if (fm1_t[cf_tri32(L, i + 1, j)] > CF_HALF) {
+                } else if (wun > CF_HALF) {
                     push(3, i + 1, j);
                 }
             }
