@@ -196,13 +196,15 @@ int launch_post(cf_batch* b, const Bucket& bk, hipStream_t st) {
     // Eight per sequence: 27.9 ms at 10,000 x 200 nt, against 30.3 for four
     // and 41 for two.  A wide bucket's workgroup width (up to 16 waves) splits
     // long folds further, as far as the per-wave LDS areas fit.
+    if (bk.Lmax >= 8192) return fail(c, CF_EINVAL, "sequence too long for the posterior kernel (8191 nt at most)");
     int split = std::max(8, bk.threads / 64);
-    CfLds plan = cf_lds_plan(bk.Lmax, false, 4, split);
-    while (plan.total > c->lds_max && split > kPack) plan = cf_lds_plan(bk.Lmax, false, 4, --split);
-    if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the posterior kernel's LDS plan");
-    hipLaunchKernelGGL(cf_post_kernel, dim3(nl), dim3(64 * split), plan.total, st, b->d_seqs, b->d_order + bk.begin,
-                       nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->F5, b->F5o, b->FCo, b->FM1o, b->POST, bk.Lmax,
-                       split);
+    const size_t seq_lds = cf_lds_plan(bk.Lmax, false, 0, 1).total;
+    auto lds = [&](int w) { return seq_lds + (size_t)w * CF_POST_WORK; };
+    while (lds(split) > (size_t)c->lds_max && split > 1) --split;
+    if (lds(split) > (size_t)c->lds_max)
+        return fail(c, CF_EINVAL, "sequence too long for the posterior kernel's LDS plan");
+    hipLaunchKernelGGL(cf_post_kernel, dim3(nl), dim3(64 * split), lds(split), st, b->d_seqs, b->d_order + bk.begin,
+                       nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->F5, b->F5o, b->FCo, b->FM1o, b->POST, bk.Lmax);
     CF_HIP(c, hipGetLastError(), "posterior launch");
     return CF_OK;
 }

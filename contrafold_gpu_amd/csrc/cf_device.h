@@ -150,8 +150,11 @@ __constant__ const float cf_exp_coef[6][4] = {
 };
 __device__ __attribute__((noinline)) float cf_fexp_tail(float x) { return (x > 46.052f) ? 1e20f : expf(x); }
 __device__ __forceinline__ float cf_fexp(float x, const float4* __restrict__ E) {
-    int seg = (x >= -5.8622823336f ? 1 : 0) + (x >= -3.8396630909f ? 1 : 0) + (x >= -2.4915033807f ? 1 : 0) +
-              (x >= -1.4805375919f ? 1 : 0) + (x >= -0.6725053211f ? 1 : 0);
+    // seg = number of breakpoints <= x, by a two-level tree on the middle one
+    const bool c3 = x >= -2.4915033807f;
+    const bool c2 = x >= (c3 ? -1.4805375919f : -5.8622823336f);
+    const bool c1 = x >= (c3 ? -0.6725053211f : -3.8396630909f);
+    const int seg = (c3 ? 3 : 0) + (c2 ? 1 : 0) + (c1 ? 1 : 0);
     const float4 c = E[seg];
     float r = c.x * x;
     r = r + c.y;

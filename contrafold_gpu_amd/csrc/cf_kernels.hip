@@ -1004,30 +1004,42 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
 // POST(a, b+1) sums exp(outside + inside - logZ) over the outer pairs that
 // close (a, b+1) as a single-branch loop, the multiloop (FM1) source and the
 // exterior source, in the reference's order.  Unlike FCo, the cells are
-// independent once the outside sweep is done, so each wave walks all cells of
-// its sequence with no synchronisation: the pairable ones are compacted into
-// a queue by ballot and evaluated a full wave (64 cells) at a time.  With
-// split > 1 the waves of a workgroup share one sequence and take every
-// split-th diagonal (lower latency for a lone long fold).
+// independent once the outside sweep is done.  A workgroup takes one
+// sequence (staged once in LDS) and its waves take every G-th diagonal; each
+// lane evaluates one cell, walking its single-branch window with the reads of
+// 8 sources in flight at a time.
+//
+// A wave runs until its busiest lane is done, and window sizes differ a lot
+// (boundary cells, base composition, masks), so each wave pools up to
+// CF_POST_POOL pairable cells, counts every pooled cell's candidates
+// (popcounts of its window rows) and takes its batches of 64 in order of that
+// count: lanes of one batch then finish together.
 // POSTF is indexed by FC cell: POSTF[tri(a, b)] = P(a pairs b+1).
+#define CF_POST_POOL 256
+// per-wave work area: the pool (cells as a | s << 13 | count class << 26)
+// with room for one more diagonal chunk, and the pool sorted by class
+#define CF_POST_WORK ((CF_POST_POOL + 64) * 4 + CF_POST_POOL * 4)
+
 __global__ void __launch_bounds__(1024) cf_post_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ order, int nseq, const uint8_t* __restrict__ codes,
     const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
     const float* __restrict__ F5, const float* __restrict__ F5o, const float* __restrict__ FCo,
-    const float* __restrict__ FM1o, float* __restrict__ POSTF, int Lmax, int split) {
+    const float* __restrict__ FM1o, float* __restrict__ POSTF, int Lmax) {
     extern __shared__ __align__(16) unsigned char smem[];
+    if ((int)blockIdx.x >= nseq) return;
     const int G = (int)(blockDim.x / WAVE);
-    const CfTeam tm = cf_team<true>(G);
-    const CfLds plan = cf_lds_plan(Lmax, false, 4, G);
-    const CfShared S = cf_carve(smem, plan, tm.team);
+    const CfLds plan = cf_lds_plan(Lmax, false, 0, 1);
+    const CfShared S = cf_carve(smem, plan, 0);
     cf_stage_params(S, P);
     __syncthreads();
-    const int li = (split > 1) ? (int)blockIdx.x : tm.li;
-    if (li >= nseq) return;
-    const CfSeq sd = seqs[order[li]];
+    const CfSeq sd = seqs[order[blockIdx.x]];
     const int L = sd.L;
-    const int lane = tm.tid;
-    cf_stage_seq<true>(S, codes, masks, sd, lane, WAVE);
+    cf_stage_seq<false>(S, codes, masks, sd, threadIdx.x, blockDim.x);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int lane = threadIdx.x & (WAVE - 1);
+    unsigned char* work = smem + plan.total + wv * CF_POST_WORK;
+    int* __restrict__ pool = (int*)work;
+    int* __restrict__ sorted = pool + CF_POST_POOL + 64;
 
     const float mp = P->t[CF_O_MP], ep = P->t[CF_O_EP];
     const float* __restrict__ T = P->t;
@@ -1038,12 +1050,25 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
     const float* __restrict__ f5o = F5o + sd.coff;
     float* __restrict__ post_t = POSTF + sd.toff;
     const float4* __restrict__ E = S.ex;
-    int* __restrict__ queue = (int*)S.dbuf;
     const float Z = f5[L];
+    const int off_neg = S.dtab[L];  // FCo(0, L): no pair (0, L+1), always NEG
+
+    auto window = [&](int a, int qbase, int l1) -> uint32_t {
+        return cf_window(S.pmask + S.rs8[a - 1 - l1] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
+    };
 
     auto cell = [&](int a, int s) {
         const int b = a + s;
+        const int lmax = min(CF_CMAX, a - 2);
+        const int qbase = b + 2;
+        // the cell's own reads are issued together
+        const uint32_t bits0 = (lmax >= 0) ? window(a, qbase, 0) : 0u;
+        const bool fm1dom = (a - 1 > 0) && (a + 1 <= b + 1) && (b + 1 < L);
         const float fcab = fc_t[S.dtab[s] + a];
+        const float fm1o_src = fm1dom ? fm1o_t[S.dtab[s + 2] + a - 1] : CF_NEG;
+        const float fco_stk = (bits0 & 1u) ? cf_ld(fco_t, S.dtab[s + 2] + a - 1) : CF_NEG;
+        const float outx = f5o[b + 1] - Z;
+        const float f5a = f5[a - 1];
         float post = 0.0f;
         if (fcab > CF_HALF) {
             const int sa = S.s8[a], sb1 = S.s8[b + 1], sam = S.s8[a - 1], sb2 = S.s8[b + 2];
@@ -1052,27 +1077,17 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
             if (b + 1 < L) jAx = jAx + T[CF_O_DL + cf_i3(sb1, sa, sb2)];
             if (a - 1 > 0) jAx = jAx + T[CF_O_DR + cf_i3(sb1, sa, sam)];
             const float jB2 = S.JB[cf_i4(sb1, sa, sb2, sam)];
-            const float stk = T[CF_O_STACK + cf_i4(sam, sb2, sa, sb1)];
-            const float sn01 = T[CF_O_B01 + sb2], sn10 = T[CF_O_B01 + sam], sn11 = T[CF_O_I11 + cf_i2(sam, sb2)];
-            const bool fm1dom = (a - 1 > 0) && (a + 1 <= b + 1) && (b + 1 < L);
-            const float fm1o_src = fm1dom ? fm1o_t[S.dtab[s + 2] + a - 1] : CF_NEG;
-            const int lmax = min(CF_CMAX, a - 2);
-            const int qbase = b + 2;
             // rows l1 = 0 .. lmax, partners ascending (the reverse of the FCo
-            // order); in row 0 the FM1o source follows the stacking term k = 0.
-            // The other sources are gathered 8 at a time (their loads do not
-            // depend on the running sum), then added in order; empty slots
-            // add nothing.
+            // order); in row 0 the FM1o source follows the stacking term k = 0
             if (lmax >= 0) {
-                uint32_t bits = cf_window(S.pmask + S.rs8[a - 1] * S.W, qbase, min(L, qbase + CF_CMAX));
-                if (bits & 1u) {
-                    const float outside = cf_ld(fco_t, S.dtab[s + 2] + a - 1) - Z;
+                if (bits0 & 1u) {
+                    const float stk = T[CF_O_STACK + cf_i4(sam, sb2, sa, sb1)];
+                    const float outside = fco_stk - Z;
                     float e = outside + bpab;
                     e = e + stk;
                     e = e + fcab;
                     post = post + ((outside > CF_HALF) ? cf_fexp(e, E) : 0.0f);
                 }
-                bits &= ~1u;
                 if (fm1o_src > CF_HALF) {
                     float v = fm1o_src + fcab;
                     v = v + jAx;
@@ -1081,51 +1096,58 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
                     v = v - Z;
                     post = post + cf_fexp(v, E);
                 }
-                int l1 = 0, ip = a - 1;
-                bool more = true;
-                while (more) {
-                    int off[8], tag[8];
+                // Sources gathered 8 at a time (their reads do not depend on
+                // the running sum), then added in order; an empty slot reads
+                // the NEG cell and adds nothing.  Rows 0 and 1 carry the
+                // 0x1 / 1x0 bulge and 1x1 extras; rows >= 2 add nothing more.
+                const float sn01 = T[CF_O_B01 + sb2], sn10 = T[CF_O_B01 + sam], sn11 = T[CF_O_I11 + cf_i2(sam, sb2)];
+                // A slot holds its FCo index and, packed, its JB index (10
+                // bits) and cs index l1*31 + kk (10 bits); the extras are at
+                // cs indices 1 (0x1), 31 (1x0) and 32 (1x1).
+                auto rows = [&](auto extras, int r0, int r1, uint32_t bits) {
+                    int l1 = r0;
+                    int rowJ = S.s8[a - 1 - l1] * 125 + S.s8[a - l1] * 5;
+                    bool more = true;
+                    while (more) {
+                        int off[8], tag[8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        while (bits == 0u && l1 < lmax) {
-                            ++l1;
-                            ip = a - 1 - l1;
-                            bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
+                        for (int u = 0; u < 8; ++u) {
+                            while (bits == 0u && l1 < r1) {
+                                ++l1;
+                                bits = window(a, qbase, l1);
+                                rowJ = S.s8[a - 1 - l1] * 125 + S.s8[a - l1] * 5;
+                            }
+                            off[u] = off_neg;
+                            tag[u] = 0;
+                            if (bits != 0u) {
+                                const int kk = __ffs(bits) - 1;
+                                bits &= bits - 1u;
+                                off[u] = S.dtab[s + l1 + kk + 2] + a - 1 - l1;
+                                tag[u] = (rowJ + S.qo[qbase + kk - 1]) | ((l1 * 31 + kk) << 10);
+                            }
                         }
-                        off[u] = 0;
-                        tag[u] = -1;
-                        if (bits != 0u) {
-                            const int kk = __ffs(bits) - 1;
-                            bits &= bits - 1u;
-                            off[u] = S.dtab[s + l1 + kk + 2] + ip;
-                            tag[u] = (l1 << 5) | kk;
+                        more = (bits != 0u) || (l1 < r1);
+                        float o[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) o[u] = cf_ld(fco_t, off[u]);
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int csi = tag[u] >> 10;
+                            const float outside = o[u] - Z;
+                            float e = outside + S.JB[tag[u] & 1023];
+                            e = e + S.cs[csi];
+                            e = e + fcab;
+                            e = e + bpab;
+                            e = e + jB2;
+                            if constexpr (decltype(extras)::value)
+                                e = e + (csi == 1 ? sn01 : csi == 31 ? sn10 : csi == 32 ? sn11 : 0.0f);
+                            post = post + ((outside > CF_HALF) ? cf_fexp(e, E) : 0.0f);
                         }
                     }
-                    more = (bits != 0u) || (l1 < lmax);
-                    float o[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) o[u] = cf_ld(fco_t, off[u]);
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        if (tag[u] < 0) continue;
-                        const int r1 = tag[u] >> 5, kk = tag[u] & 31;
-                        const int ipr = a - 1 - r1;
-                        const int rowJ = S.s8[ipr] * 125 + S.s8[ipr + 1] * 5;
-                        const float outside = o[u] - Z;
-                        float e = outside + S.JB[rowJ + S.qo[qbase + kk - 1]];
-                        e = e + S.cs[r1 * 31 + kk];
-                        e = e + fcab;
-                        e = e + bpab;
-                        e = e + jB2;
-                        const float sn = (r1 == 0) ? (kk == 1 ? sn01 : 0.0f)
-                                       : (r1 == 1) ? (kk == 0 ? sn10 : (kk == 1 ? sn11 : 0.0f)) : 0.0f;
-                        e = e + sn;
-                        post = post + ((outside > CF_HALF) ? cf_fexp(e, E) : 0.0f);
-                    }
-                }
+                };
+                rows(std::true_type{}, 0, min(lmax, 1), bits0 & ~1u);
+                if (lmax >= 2) rows(std::false_type{}, 2, lmax, window(a, qbase, 2));
             }
-            const float outx = f5o[b + 1] - Z;
-            const float f5a = f5[a - 1];
             if (outx > CF_HALF && f5a > CF_HALF) {
                 float v = outx + f5a;
                 v = v + fcab;
@@ -1139,9 +1161,53 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
         post_t[S.dtab[s] + a] = post;
     };
 
+    // candidates of a pooled cell, as a count class (16 per class)
+    auto count_class = [&](int a, int s) -> int {
+        const int lmax = min(CF_CMAX, a - 2);
+        const int qbase = a + s + 2;
+        int n = 0;
+        for (int l1 = 0; l1 <= lmax; ++l1) n += __popc(window(a, qbase, l1));
+        return min(n >> 4, 31);
+    };
+
+    // Sort the first n pooled cells by class (counting sort over the wave:
+    // per class, ballots give each member its slot) and evaluate them 64 at a
+    // time; lane l takes the l-th cell of each batch.
+    auto drain = [&](int n) {
+        constexpr int R = CF_POST_POOL / WAVE;
+        int ent[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int t = r * WAVE + lane;
+            ent[r] = -1;
+            if (t < n) {
+                const int e = pool[t];
+                ent[r] = e | (count_class(e & 0x1fff, (e >> 13) & 0x1fff) << 26);
+            }
+        }
+        int base = 0;
+        for (int c = 0; c < 32; ++c) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bool hit = ent[r] >= 0 && (ent[r] >> 26) == c;
+                const uint64_t m = __ballot(hit);
+                if (hit) sorted[base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = ent[r];
+                base += __popcll(m);
+            }
+        }
+        cf_team_sync<true>();
+        for (int t0 = 0; t0 < n; t0 += WAVE) {
+            if (t0 + lane < n) {
+                const int e = sorted[t0 + lane];
+                cell(e & 0x1fff, (e >> 13) & 0x1fff);
+            }
+        }
+        cf_team_sync<true>();
+    };
+
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int qn = 0;  // queued cells (wave-uniform)
-    for (int s = (split > 1) ? tm.team : 0; s <= L; s += split) {
+    int pn = 0;  // pooled cells (wave-uniform)
+    for (int s = wv; s <= L; s += G) {
         const int ncell = L + 1 - s;
         for (int i0 = 0; i0 < ncell; i0 += WAVE) {
             const int a = i0 + lane;
@@ -1150,26 +1216,21 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
             const bool pr = live && (a > 0) && (b < L) && cf_allow(S, a, b + 1);
             if (live && !pr) post_t[S.dtab[s] + a] = 0.0f;
             const uint64_t m = __ballot(pr);
-            if (pr) queue[qn + __popcll(m & below)] = (a << 16) | s;
-            qn += __popcll(m);
-            if (qn >= WAVE) {
+            if (pr) pool[pn + __popcll(m & below)] = a | (s << 13);
+            pn += __popcll(m);
+            if (pn >= CF_POST_POOL) {
                 cf_team_sync<true>();
-                const int e = queue[lane];
-                cell(e >> 16, e & 0xffff);
+                drain(CF_POST_POOL);
+                const int rest = pn - CF_POST_POOL;
+                const int moved = (lane < rest) ? pool[CF_POST_POOL + lane] : 0;
                 cf_team_sync<true>();
-                const int rest = qn - WAVE;
-                const int moved = (lane < rest) ? queue[WAVE + lane] : 0;
-                cf_team_sync<true>();
-                if (lane < rest) queue[lane] = moved;
-                qn = rest;
+                if (lane < rest) pool[lane] = moved;
+                pn = rest;
             }
         }
     }
     cf_team_sync<true>();
-    if (lane < qn) {
-        const int e = queue[lane];
-        cell(e >> 16, e & 0xffff);
-    }
+    if (pn > 0) drain(pn);
 }
 
 // ======================================================================= MEA
