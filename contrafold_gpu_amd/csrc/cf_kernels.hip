@@ -879,79 +879,76 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
             // Sources in the reference's order: outer rows i' = a-1-l1 ascending
             // (l1 descending), partners descending; in row l1 = 0 the FM1o source
             // of iteration (a-1, b+1) comes after every k >= 1 and before k = 0.
-            // Rows l1 >= 2 (no stacking, bulge or 1x1 extras) stream 8 sources at
-            // a time -- their reads do not depend on acc; exhausted slots fold
-            // NEG, which leaves acc unchanged -- then rows 1 and 0 follow one
-            // source at a time.
-            auto term = [&](int r1, int ipr, int rJ, int l2) -> float {
-                const int dd = d + r1 + l2 + 2;
-                const float fco = RING ? ring[(dd % CF_RING) * RS + ipr] : cf_ld(fco_t, S.dtab[dd] + ipr);
-                float e = S.cs[r1 * 31 + l2] + bpab;
-                e = e + S.JB[rJ + S.qo[qbase + l2 - 1]];
-                e = e + jB2;
-                const float sn = (r1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
-                               : (r1 == 1) ? (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f)) : 0.0f;
-                e = e + sn;
-                e = (r1 + l2 == 0) ? e_stack : e;
-                return (fco > CF_HALF) ? fco + e : CF_NEG;
-            };
-            if (lmax >= 2) {
-                int l1 = lmax;
-                int ip = a - 1 - l1;
-                uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
-                int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
+            // Sources stream 8 at a time -- their reads do not depend on acc;
+            // exhausted slots fold NEG, which leaves acc unchanged.  Rows l1 >= 2
+            // add the reference's zero extra; rows 1 and 0 (k >= 1) carry the
+            // bulge 0x1 / 1x0 and 1x1 extras and are walked by their own
+            // instance; the stacking source k = 0 of row 0 is read up front.
+            // Without the ring the FCo loads of a batch are consumed only in
+            // its fold, so all 8 are in flight together; an empty slot reads
+            // cell (0, L), set to NEG by the first step, and its term is NEG.
+            const int off_neg = S.dtab[L];
+            const uint32_t bits_r0 = (lmax >= 0) ? cf_window(S.pmask + S.rs8[a - 1] * S.W, qbase, min(L, qbase + CF_CMAX))
+                                                 : 0u;
+            float fco_stk = CF_NEG;
+            if (bits_r0 & 1u) {
+                if constexpr (RING) fco_stk = ring[((d + 2) % CF_RING) * RS + a - 1];
+                else fco_stk = cf_ld(fco_t, S.dtab[d + 2] + a - 1);
+            }
+            const int16_t* qor = S.qo + (qbase - 1);
+            auto walk = [&](auto extras, int l1, int l1_last) {
+                int ip = 0, rowJ = 0;
+                uint32_t bits = 0u;
+                const int* dtr = S.dtab;
+                const float* csr = S.cs;
                 // per row, candidate l2 indexes dtab[d + l1 + 2 + l2] and cs[l1*31 + l2]
-                const int* dtr = S.dtab + (d + l1 + 2);
-                const float* csr = S.cs + l1 * 31;
-                const int16_t* qor = S.qo + (qbase - 1);
-                // Without the ring the FCo loads of a batch are consumed only in
-                // its fold, so all 8 are in flight together; an empty slot reads
-                // cell (0, L), set to NEG by the first step, and its term is NEG.
-                const int off_neg = S.dtab[L];
+                auto row = [&]() {
+                    ip = a - 1 - l1;
+                    bits = (l1 == 0) ? (bits_r0 & ~1u)
+                                     : cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
+                    rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
+                    dtr = S.dtab + (d + l1 + 2);
+                    csr = S.cs + l1 * 31;
+                };
+                if (l1 < l1_last) return;
+                row();
                 bool more = true;
                 while (more) {
                     float fv[8], ev[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        while (bits == 0u && l1 > 2) {
+                        while (bits == 0u && l1 > l1_last) {
                             --l1;
-                            ip = a - 1 - l1;
-                            bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
-                            rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
-                            dtr = S.dtab + (d + l1 + 2);
-                            csr = S.cs + l1 * 31;
+                            row();
                         }
                         float e = 0.0f;
+                        int off = off_neg;
+                        int l2 = 0;
+                        const bool has = bits != 0u;
+                        if (has) {
+                            l2 = 31 - __clz(bits);
+                            bits &= ~(1u << l2);
+                            off = dtr[l2] + ip;
+                            e = csr[l2] + bpab;
+                            e = e + S.JB[rowJ + qor[l2]];
+                            e = e + jB2;
+                            if constexpr (decltype(extras)::value) {
+                                const float sn = (l1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
+                                               : (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f));
+                                e = e + sn;
+                            } else {
+                                e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
+                            }
+                        }
                         if constexpr (RING) {
-                            float val = CF_NEG;
-                            if (bits != 0u) {
-                                const int l2 = 31 - __clz(bits);
-                                bits &= ~(1u << l2);
-                                const int dd = d + l1 + l2 + 2;
-                                const float fco = ring[(dd % CF_RING) * RS + ip];
-                                e = csr[l2] + bpab;
-                                e = e + S.JB[rowJ + qor[l2]];
-                                e = e + jB2;
-                                e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
-                                val = (fco > CF_HALF) ? fco + e : CF_NEG;
-                            }
-                            fv[u] = val;
+                            const float fco = has ? ring[((d + l1 + l2 + 2) % CF_RING) * RS + ip] : CF_NEG;
+                            fv[u] = (fco > CF_HALF) ? fco + e : CF_NEG;
                         } else {
-                            int off = off_neg;
-                            if (bits != 0u) {
-                                const int l2 = 31 - __clz(bits);
-                                bits &= ~(1u << l2);
-                                off = dtr[l2] + ip;
-                                e = csr[l2] + bpab;
-                                e = e + S.JB[rowJ + qor[l2]];
-                                e = e + jB2;
-                                e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
-                            }
                             fv[u] = cf_ld(fco_t, off);
                             ev[u] = e;
                         }
                     }
-                    more = (bits != 0u) || (l1 > 2);
+                    more = (bits != 0u) || (l1 > l1_last);
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         if constexpr (RING)
@@ -960,27 +957,17 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
                             acc = cf_lse(acc, (fv[u] > CF_HALF) ? fv[u] + ev[u] : CF_NEG, C);
                     }
                 }
-            }
-            for (int l1 = min(lmax, 1); l1 >= 0; --l1) {
-                const int ip = a - 1 - l1;
-                uint32_t bits = cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
-                const int rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
-                const uint32_t last = (l1 == 0) ? (bits & 1u) : 0u;
-                if (l1 == 0) bits &= ~1u;
-                while (bits != 0u) {
-                    const int kk = 31 - __clz(bits);
-                    bits &= ~(1u << kk);
-                    acc = cf_lse(acc, term(l1, ip, rowJ, kk), C);
+            };
+            if (lmax >= 2) walk(std::false_type{}, lmax, 2);
+            if (lmax >= 0) {
+                walk(std::true_type{}, min(lmax, 1), 0);
+                if (fm1o_src > CF_HALF) {
+                    float v = fm1o_src + jAx;
+                    v = v + mp;
+                    v = v + bpab;
+                    acc = cf_lse(acc, v, C);
                 }
-                if (l1 == 0) {
-                    if (fm1o_src > CF_HALF) {
-                        float v = fm1o_src + jAx;
-                        v = v + mp;
-                        v = v + bpab;
-                        acc = cf_lse(acc, v, C);
-                    }
-                    if (last) acc = cf_lse(acc, term(0, ip, rowJ, 0), C);
-                }
+                if (bits_r0 & 1u) acc = cf_lse(acc, (fco_stk > CF_HALF) ? fco_stk + e_stack : CF_NEG, C);
             }
             fco_t[db + a] = acc;
             if (RING) ring[(d % CF_RING) * RS + a] = acc;
