@@ -4,7 +4,9 @@
 
 hipcc compiles the HIP sources for gfx950 only, with -ffp-contract=off (every
 multiply-add is two rounded operations, as in the reference's float32 engine)
-and with packed-FP32 instructions disabled.  The library is written in-tree
+with packed-FP32 instructions disabled, and with -fno-honor-nans (no value
+the kernels handle is ever NaN -- an empty cell is -1e30 -- so min/max need no
+NaN quieting).  The library is written in-tree
 next to this file so that it travels with the source tree.
 """
 import os
@@ -30,6 +32,7 @@ def hipcc():
 
 def flags():
     return ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+            "-fno-honor-nans",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I" + os.path.join(os.path.dirname(HERE), "include"), "-I" + CSRC,
             "-Wno-unused-result", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]

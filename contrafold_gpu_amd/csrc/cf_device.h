@@ -121,8 +121,9 @@ __device__ __forceinline__ float cf_lse(float a, float b, const float4* __restri
     const bool b1 = d >= (b2 ? 5.7890710412f : 1.6320158198f);
     const float t0 = b2 ? (b1 ? 7.8162726752f : 4.4261691294f) : (b1 ? 2.4912588184f : 0.6615367791f);
     const bool b0 = d >= t0;
-    const int seg = (b2 ? 4 : 0) | (b1 ? 2 : 0) | (b0 ? 1 : 0);
-    const float4 c = C[seg];
+    // byte offset of the interval's coefficients (three selects and one add)
+    const int off = (b2 ? 64 : 0) + (b1 ? 32 : 0) + (b0 ? 16 : 0);
+    const float4 c = *(const float4*)((const char*)C + off);
     float r = c.x * d;
     r = r + c.y;
     r = r * d;
@@ -154,8 +155,8 @@ __device__ __forceinline__ float cf_fexp(float x, const float4* __restrict__ E) 
     const bool c3 = x >= -2.4915033807f;
     const bool c2 = x >= (c3 ? -1.4805375919f : -5.8622823336f);
     const bool c1 = x >= (c3 ? -0.6725053211f : -3.8396630909f);
-    const int seg = (c3 ? 3 : 0) + (c2 ? 1 : 0) + (c1 ? 1 : 0);
-    const float4 c = E[seg];
+    const int off = (c3 ? 48 : 0) + (c2 ? 16 : 0) + (c1 ? 16 : 0);  // byte offset of E[seg]
+    const float4 c = *(const float4*)((const char*)E + off);
     float r = c.x * x;
     r = r + c.y;
     r = r * x;
