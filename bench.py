@@ -323,9 +323,10 @@ def main():
             if name.startswith(f"cf_{dominant}_kernel"):
                 roof["traffic"] = rec["bytes_per_launch"]
                 roof["traffic_unit"] = "bytes/launch"
+    # the stages this step runs (the other event pairs only time an empty gap)
     per_kernel = {k: {"ms": round(v, 3),
                       "tflops": (round(flops[k] / (v * 1e-3) / 1e12, 3) if k in flops and v > 0 else None)}
-                  for k, v in kavg.items()}
+                  for k, v in kavg.items() if k in ("inside", "exterior", "outside", "post", "sample")}
 
     # MEA headline on the same resident batch (inside/outside recomputed)
     batch.run(cf.CF_MEA, gammas=[6.0], recompute=True)
