@@ -65,6 +65,42 @@ def test_sample_masks_and_validity(P):
                 assert (s[a], s[b]) in CANON
 
 
+def test_sample_long_sequences(P):
+    """Sequences past the sampler's LDS-resident form (a batch whose longest
+    sequence does not fit the per-wave LDS slice reads F5 rows and codes from
+    HBM): a 2,338-nt repeat, a masked 700-nt sequence and a short one.  Samples
+    are nested canonical structures that respect the mask, and the masked
+    sequence's pair frequencies follow bpp within binomial noise.  (Stacks
+    deeper than the 128 LDS-resident entries continue in HBM; sampled
+    structures of even 1,700 nt need only 10-15 entries, so no test reaches
+    that spill.)"""
+    import contrafold_gpu_amd as cf
+    rng = np.random.default_rng(11)
+    rep = "GGGAAACCCAAAAA" * 167
+    long_masked = rand_seq(rng, 700)
+    mask = (rng.random(700) < 0.1).astype(np.int8)
+    seqs = [rep, long_masked, rand_seq(rng, 60)]
+    masks = [np.zeros(len(rep), np.int8), mask, np.zeros(60, np.int8)]
+    n = 400
+    res = cf.sample_batch(seqs, P, n, forced_list=masks, seed=5)
+    for s, m, smp in zip(seqs, masks, res):
+        assert len(smp) == n
+        for db in smp:
+            assert len(db) == len(s)
+            for a, b in pairs_of(db):
+                assert not m[a] and not m[b]
+                assert (s[a], s[b]) in CANON
+    L = len(long_masked)
+    freq = np.zeros((L, L))
+    for db in res[1]:
+        for a, b in pairs_of(db):
+            freq[a, b] += 1
+    freq /= n
+    post = cf.bpp(long_masked, P, forced=mask)
+    sigma = np.sqrt(post * (1 - post) / n)
+    assert (np.abs(freq - post) <= 5 * sigma + 0.02).all()
+
+
 def test_sampling_batch_invariant(P):
     import contrafold_gpu_amd as cf
     rng = np.random.default_rng(4)
