@@ -67,6 +67,7 @@ struct cf_ctx {
     // bucket edges (max length) and team width per bucket; width 64 = packed
     std::vector<int> edges{128, 384, 1 << 30};
     std::vector<int> threads{64, 64, 512};
+    bool tuned = false;  // edges / widths set by cf_set_tuning (else the built-in default)
     int lds_max = 65536;
     int n_cu = 256;
     cfi::DevPool pool;
