@@ -301,7 +301,11 @@ def main():
     wc = workcount.batch_counts(seqs, masks, sample=24)
     flops = {"inside": 9.0 * wc["inside"], "exterior": 9.0 * wc["exterior"],
              "outside": 9.0 * wc["outside"], "post": 9.0 * wc["posterior"], "mea": 2.0 * wc["mea"]}
-    dominant = max(kavg, key=kavg.get)
+    # The roofline kernel is the longest stage of the batch stream.  The sampler
+    # runs on a lane stream beside the outside sweep and the posterior (it needs
+    # only the inside tables), so its marks span their time too and do not
+    # measure it alone.
+    dominant = max((k for k in kavg if k in flops), key=kavg.get)
     roof = None
     if dominant in flops and kavg[dominant] > 0:
         ach = flops[dominant] / (kavg[dominant] * 1e-3) / 1e12
@@ -350,6 +354,8 @@ def main():
                    "batch": args.n, "seq_len": args.length, "samples_per_seq": args.samples},
         "roofline": roof,
         "kernels": per_kernel,
+        "kernels_note": "device ms per stage (HIP events); the sampler runs on a lane stream beside the "
+                        "outside sweep and the posterior, so those three overlap",
         "mea": {"folds_per_s": round(mea_fps, 2), "gamma": 6.0},
         "published_workloads": published,
     }

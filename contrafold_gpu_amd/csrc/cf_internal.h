@@ -127,7 +127,10 @@ struct cf_batch {
     double* grad = nullptr;
     unsigned done = 0;
     uint64_t params_version = 0;
-    hipEvent_t ev[cfi::kNumKernels + 3] = {};  // stage marks 0..8, run start, run end
+    // stage marks 0..8, run start, run end, and the start / end of a sampler
+    // launched on a lane stream beside the outside sweep
+    hipEvent_t ev[cfi::kNumKernels + 5] = {};
+    bool sample_lane = false;  // the last run sampled on lanes[0]
     float last_ms = 0.0f;
     float kms[cfi::kNumKernels] = {};
     // per-bucket streams of an eager multi-bucket run (see enqueue_run)

@@ -230,6 +230,32 @@ def test_bucket_parts_match_serial(P):
                 assert (b.bpp(k).view(np.uint32) == m.view(np.uint32)).all()
 
 
+def test_sampler_beside_outside_matches_serial(P):
+    """A run with logZ, bpp and samples launches the sampler on a lane stream
+    beside the outside sweep and the posterior (eager, timed and under graph
+    capture); its samples equal those of a logZ + samples run, where it runs
+    alone on the batch stream, and its bpp equals a bpp-only run."""
+    from contrafold_gpu_amd import CF_BPP, CF_LOGZ, CF_SAMPLE, Engine
+    from contrafold_gpu_amd._lib import CF_GRAPH
+    rng = np.random.default_rng(29)
+    seqs = [rand_seq(rng, 90) for _ in range(300)]
+    eng = Engine()
+    eng.set_params(P)
+    with eng.batch(seqs) as b:
+        b.run(CF_LOGZ | CF_SAMPLE, n_samples=4, seed=3, recompute=True)
+        ref_s = b.samples()
+        b.run(CF_BPP, recompute=True)
+        ref_b = [b.bpp(k).copy() for k in (0, 150, 299)]
+        for kw in ({}, {"timing": True}, {"graph": True}, {"graph": True}):
+            flags = CF_BPP | CF_SAMPLE | (CF_GRAPH if kw.get("graph") else 0)
+            b.run(flags, n_samples=4, seed=3, recompute=True, timing=kw.get("timing", False))
+            assert b.samples() == ref_s
+            for k, m in zip((0, 150, 299), ref_b):
+                assert (b.bpp(k).view(np.uint32) == m.view(np.uint32)).all()
+        b.run(CF_BPP | CF_SAMPLE, n_samples=4, seed=3, recompute=True, timing=True)
+        assert b.kernel_ms()["sample"] > 0
+
+
 def test_launch_shape_invariance(P):
     """Results are bit-identical across bucket edges and workgroup sizes (a
     cross-lane race would show up as a mismatch)."""
