@@ -107,7 +107,9 @@ int cf_batch_table(const cf_batch *b, int index, int which, float *out);
 /* Device milliseconds of the last cf_batch_run (HIP events on the batch stream). */
 float cf_batch_last_ms(const cf_batch *b);
 /* Per-kernel device milliseconds of the last CF_TIMING run; names in
- * cf_kernel_name(k), k < cf_kernel_count(). */
+ * cf_kernel_name(k), k < cf_kernel_count().  In a run with logZ, bpp and
+ * samples the sampler runs on a lane stream beside the outside sweep and the
+ * posterior, so those three stages overlap. */
 int cf_kernel_count(void);
 const char *cf_kernel_name(int k);
 int cf_batch_kernel_ms(const cf_batch *b, float *out);
