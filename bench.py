@@ -327,6 +327,11 @@ def main():
             if name.startswith(f"cf_{dominant}_kernel"):
                 roof["traffic"] = rec["bytes_per_launch"]
                 roof["traffic_unit"] = "bytes/launch"
+                # the same launch seen from HBM: measured bytes over its duration
+                if kavg.get(dominant, 0) > 0:
+                    gbs = rec["bytes_per_launch"] / (kavg[dominant] * 1e-3) / 1e9
+                    roof["traffic_gbs"] = round(gbs, 1)
+                    roof["traffic_frac_of_hbm_peak"] = round(gbs / HBM_PEAK_GBS, 4)
     # the stages this step runs (the other event pairs only time an empty gap)
     per_kernel = {k: {"ms": round(v, 3),
                       "tflops": (round(flops[k] / (v * 1e-3) / 1e12, 3) if k in flops and v > 0 else None)}
@@ -353,6 +358,7 @@ def main():
                                f"Boltzmann samples/seq (Philox)",
                    "batch": args.n, "seq_len": args.length, "samples_per_seq": args.samples},
         "roofline": roof,
+        "cells_per_s": round(args.n * (args.length + 1) * (args.length + 2) / 2 / (wall / args.steps), 1),
         "kernels": per_kernel,
         "kernels_note": "device ms per stage (HIP events); the sampler runs on a lane stream beside the "
                         "outside sweep and the posterior, so those three overlap",
