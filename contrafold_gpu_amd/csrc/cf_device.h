@@ -138,9 +138,11 @@ __device__ __forceinline__ float cf_lse(float a, float b, const float4* __restri
     return (d >= 11.8624794162f) ? hi : res;
 }
 
-// CONTRAfold Fast_Exp for RealT=float (cpu.py:560-580), branch-free except
-// for the x >= 0 tail, which calls expf.  A term with x >= 0 is >= 1 and the
-// posterior it feeds is clamped to 1, so that tail never shows in a result.
+// CONTRAfold Fast_Exp for RealT=float (cpu.py:560-580), branch-free.  For
+// x >= 0 the reference calls expf (>= 1); the only consumer here is the
+// posterior, a sum of non-negative terms clamped to 1, which any term >= 1
+// sends to exactly 1, so 1 stands in for that tail and the result bits are the
+// same.
 __constant__ const float cf_exp_coef[6][4] = {
     {0.0000803850f, 0.0021627428f, 0.0194708555f, 0.0588080014f},
     {0.0013889414f, 0.0244676474f, 0.1471290604f, 0.3042757740f},
@@ -149,7 +151,6 @@ __constant__ const float cf_exp_coef[6][4] = {
     {0.0573782771f, 0.3580258429f, 0.9121133217f, 0.9793091728f},
     {0.1199175927f, 0.4815668234f, 0.9975991939f, 0.9999505077f},
 };
-__device__ __attribute__((noinline)) float cf_fexp_tail(float x) { return (x > 46.052f) ? 1e20f : expf(x); }
 __device__ __forceinline__ float cf_fexp(float x, const float4* __restrict__ E) {
     // seg = number of breakpoints <= x, by a two-level tree on the middle one
     const bool c3 = x >= -2.4915033807f;
@@ -164,7 +165,7 @@ __device__ __forceinline__ float cf_fexp(float x, const float4* __restrict__ E) 
     r = r * x;
     r = r + c.w;
     if (x < -9.91152f) r = 0.0f;
-    if (__builtin_expect(x >= 0.0f, 0)) r = cf_fexp_tail(x);
+    if (x >= 0.0f) r = 1.0f;
     return r;
 }
 
