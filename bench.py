@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector/matrix peak (dense)
 HBM_PEAK_GBS = 8000.0
+FP32_UNPACKED_TFLOPS = 78.6  # FP32 VALU issue ceiling without packed ops
 
 
 def log(msg):
@@ -55,7 +56,13 @@ def cpu_baseline(seqs, masks, P, n_samples, seconds_target=15.0):
     t = time.perf_counter()
     O.run_batch("bpp_sample32", seqs[:n], P, masks=masks[:n], threads=cores, n_samples=n_samples)
     dt = time.perf_counter() - t
+    # one core, over a few seconds of the same work
+    n1 = int(min(len(seqs), max(2, per_s / cores * 3.0)))
+    t = time.perf_counter()
+    O.run_batch("bpp_sample32", seqs[:n1], P, masks=masks[:n1], threads=1, n_samples=n_samples)
+    one_core = n1 / (time.perf_counter() - t)
     res = {"value": round(n / dt, 2), "unit": "folds/s", "cores": cores, "kind": "port",
+           "one_core_value": round(one_core, 2),
            "sample": f"first {n} of the same 200-nt sequences and masks: FP32 inside + outside + posterior "
                      f"(logZ + bpp) and {n_samples} Boltzmann samples per sequence, by the oracle (C restatement "
                      f"of gpu_contrafold/cpu.py:583-820 and gpu.py:196-477, -O3, no FMA), one fold per "
@@ -311,7 +318,9 @@ def main():
         ach = flops[dominant] / (kavg[dominant] * 1e-3) / 1e12
         roof = {"kernel": dominant, "bound": "mfma", "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 5), "traffic": None,
-                "ceiling": "FP32 VALU (no MFMA: log-space DP); peak = MI355X FP32 157.3 TFLOP/s"}
+                "ceiling": "FP32 VALU (no MFMA: log-space DP); peak = MI355X FP32 157.3 TFLOP/s",
+                # the issue ceiling without packed FP32 (256 CUs x 64 lanes x 2 x 2.4 GHz)
+                "frac_of_unpacked_fp32": round(ach / FP32_UNPACKED_TFLOPS, 5)}
     else:
         # sampling dominates: report it against HBM with the bytes it must read
         roof = {"kernel": dominant, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
