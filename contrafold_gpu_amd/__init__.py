@@ -54,7 +54,7 @@ def logZ_batch(seqs, P, forced_list=None, threads=128):
     """Log partition functions, FP32 (gpu.py:480-517).  ``threads`` is
     accepted for compatibility; launch shapes come from the tuner."""
     eng = _prep(P)
-    with eng.batch(seqs, forced_list) as b:
+    with eng.batch(seqs, forced_list, sampling=False) as b:
         b.run(CF_LOGZ)
         return [float(z) for z in b.logz()]
 
@@ -90,7 +90,7 @@ def fold_tasks_gpu(seqs, forced_masks, P, chunk=4096, seed=0):
 def bpp_batch(seqs, P, forced_list=None):
     """Base-pair probability matrices (L x L, float64, 0-based upper triangle)."""
     eng = _prep(P)
-    with eng.batch(seqs, forced_list) as b:
+    with eng.batch(seqs, forced_list, sampling=False) as b:
         b.run(CF_BPP)
         return [b.bpp(k).astype(np.float64) for k in range(b.n)]
 
@@ -106,7 +106,7 @@ def mea_batch(seqs, P, gamma=6.0, forced_list=None):
     one list of structures per gamma)."""
     gam = np.atleast_1d(np.asarray(gamma, np.float64)).astype(np.float32)
     eng = _prep(P)
-    with eng.batch(seqs, forced_list) as b:
+    with eng.batch(seqs, forced_list, sampling=False) as b:
         b.run(CF_MEA, gammas=gam)
         res = [b.mea(g) for g in range(gam.size)]
     return res[0] if np.ndim(gamma) == 0 else res
@@ -121,7 +121,7 @@ def mea(seq, P, gamma=6.0, forced=None):
 def mfe_batch(seqs, P, forced_list=None, return_scores=False):
     """Viterbi (maximum-probability) structures, FP64 (cpu.py:290-519)."""
     eng = _prep(P)
-    with eng.batch(seqs, forced_list) as b:
+    with eng.batch(seqs, forced_list, sampling=False) as b:
         b.run(CF_MFE)
         db, sc = b.mfe()
     return (db, sc) if return_scores else db
@@ -138,7 +138,7 @@ def cpu_logZ(seq, P):
     quantity of the reference's cpu.logZ (cpu.py:282-287).  Computed on the
     GPU by the FP64 exact-log1p inside pass."""
     eng = _prep(P)
-    with eng.batch([seq]) as b:
+    with eng.batch([seq], sampling=False) as b:
         b.run(CF_LOGZ64)
         return float(b.logz64()[0])
 
@@ -147,7 +147,7 @@ def logZ_grad(seqs, P, forced_list=None, weights=None):
     """sum_b w_b * d logZ_b / d table (flat 2577 vector) from the outside
     pass: expected feature counts."""
     eng = _prep(P)
-    with eng.batch(seqs, forced_list) as b:
+    with eng.batch(seqs, forced_list, sampling=False) as b:
         b.run(CF_GRAD, weights=weights)
         return b.logz(), b.grad()
 

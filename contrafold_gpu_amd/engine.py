@@ -30,11 +30,16 @@ def encode_batch(seqs, masks=None):
     seqs = list(seqs)
     if len(seqs) == 0:
         raise ValueError("empty sequence list")
-    enc = [s.encode("ascii", "replace") if isinstance(s, str) else bytes(s) for s in seqs]
-    lens = np.fromiter((len(e) for e in enc), np.int64, len(enc))
-    offs = np.zeros(len(enc) + 1, np.int64)
+    if all(type(s) is str for s in seqs):
+        # one join and one encode: "replace" maps every character to one byte
+        lens = np.fromiter(map(len, seqs), np.int64, len(seqs))
+        buf = "".join(seqs).encode("ascii", "replace")
+    else:
+        enc = [s.encode("ascii", "replace") if isinstance(s, str) else bytes(s) for s in seqs]
+        lens = np.fromiter((len(e) for e in enc), np.int64, len(enc))
+        buf = b"".join(enc)
+    offs = np.zeros(len(seqs) + 1, np.int64)
     np.cumsum(lens, out=offs[1:])
-    buf = b"".join(enc)
     mbuf = None
     if masks is not None:
         if len(masks) != len(seqs):
@@ -58,16 +63,20 @@ def sequence_keys(buf, offs, mbuf):
     occurrence counter for repeats: a sequence's samples do not depend on its
     position in the batch or on the other sequences."""
     n = len(offs) - 1
-    keys = np.empty(n, np.uint64)
     seen = {}
+    crc, adler = zlib.crc32, zlib.adler32
+    view = memoryview(buf)
+    mask = mbuf.tobytes() if mbuf is not None else None
+    bounds = offs.tolist()
+    out = [0] * n
     for k in range(n):
-        a, b = int(offs[k]), int(offs[k + 1])
-        chunk = buf[a:b] + (mbuf[a:b].tobytes() if mbuf is not None else b"")
-        h = (zlib.crc32(chunk) << 32) | zlib.adler32(chunk)
+        a, b = bounds[k], bounds[k + 1]
+        chunk = view[a:b] if mask is None else bytes(view[a:b]) + mask[a:b]
+        h = (crc(chunk) << 32) | adler(chunk)
         occ = seen.get(h, 0)
         seen[h] = occ + 1
-        keys[k] = np.uint64((h ^ (occ * 0x9E3779B97F4A7C15)) & 0xFFFFFFFFFFFFFFFF)
-    return keys
+        out[k] = (h ^ (occ * 0x9E3779B97F4A7C15)) & 0xFFFFFFFFFFFFFFFF
+    return np.array(out, dtype=np.uint64)
 
 
 class Engine:
@@ -112,27 +121,33 @@ class Engine:
         t = (_c.c_int * len(threads))(*threads)
         check(self.lib.cf_set_tuning(self.ctx, e, t, len(edges)), self.ctx, "cf_set_tuning")
 
-    def batch(self, seqs, masks=None, keys=None):
-        return Batch(self, seqs, masks, keys)
+    def batch(self, seqs, masks=None, keys=None, sampling=True):
+        return Batch(self, seqs, masks, keys, sampling)
 
 
 class Batch:
     """Device-resident fold batch (see cf_batch_* in include/contrafold_amd.h)."""
 
-    def __init__(self, engine, seqs, masks=None, keys=None):
+    def __init__(self, engine, seqs, masks=None, keys=None, sampling=True):
+        """``keys``: per-sequence sampling-stream keys; by default derived from
+        each sequence's content (``sequence_keys``).  ``sampling=False`` (a
+        batch that will not sample) skips deriving them: its streams are then
+        keyed by index."""
         self.engine = engine
         self.lib = engine.lib
         buf, offs, mbuf, lens = encode_batch(seqs, masks)
         self.lens = lens
         self.offs = offs
         self.n = len(lens)
-        if keys is None:
+        if keys is None and sampling:
             keys = sequence_keys(buf, offs, mbuf)
-        keys = np.ascontiguousarray(keys, np.uint64)
+        kptr = None
+        if keys is not None:
+            keys = np.ascontiguousarray(keys, np.uint64)
+            kptr = keys.ctypes.data_as(_c.POINTER(_c.c_uint64))
         h = _c.c_void_p()
         check(self.lib.cf_batch_create(engine.ctx, self.n, buf, offs.ctypes.data_as(_c.POINTER(_c.c_int64)),
-                                       mbuf.ctypes.data if mbuf is not None else None,
-                                       keys.ctypes.data_as(_c.POINTER(_c.c_uint64)), _c.byref(h)),
+                                       mbuf.ctypes.data if mbuf is not None else None, kptr, _c.byref(h)),
               engine.ctx, "cf_batch_create")
         self.h = h
         self.n_gamma = 0
