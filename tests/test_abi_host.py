@@ -99,6 +99,39 @@ def test_encode_batch_masks_and_errors():
     assert k1[0] == k2[1] and k1[2] != k1[0]          # content keys, repeats distinct
 
 
+def test_encode_batch_one_join_matches_per_sequence():
+    """String batches are encoded with one join: the bytes and lengths equal
+    per-sequence ASCII encoding with replacement (non-ASCII characters become
+    one '?' byte each), and bytes inputs take the per-item path."""
+    from contrafold_gpu_amd.engine import encode_batch
+    seqs = ["GGGGAAAACCCC", "acgtNNx\u00e9\u4e2dZ", "", "ACGU"]
+    buf, offs, mbuf, lens = encode_batch(seqs)
+    ref = [x.encode("ascii", "replace") for x in seqs]
+    assert buf == b"".join(ref)
+    assert list(lens) == [len(x) for x in ref]
+    assert list(offs) == [0, 12, 22, 22, 26]
+    buf, offs, _, lens = encode_batch([b"ACGU", "GGCC"])
+    assert buf == b"ACGUGGCC" and list(lens) == [4, 4]
+
+
+def test_sequence_keys_pinned():
+    """Sampling-stream keys depend only on content (and mask) plus a repeat
+    counter; pinned so that a change to the key derivation, which would change
+    every sampled structure, does not pass unnoticed."""
+    from contrafold_gpu_amd.engine import encode_batch, sequence_keys
+    seqs = ["GGGGAAAACCCC", "ACGU", "ACGU", "", "acgtn"]
+    k = sequence_keys(*encode_batch(seqs)[:3])
+    assert [int(x) for x in k] == [0x6fe83a4314ea032d, 0xd409af6402b40121, 0x4a3ed6dd7dfe7d34, 0x1,
+                                   0xe722dc050601020e]
+    masks = [None, [0, 1, 0, 0], None, None, [1, 0, 0, 0, 0]]
+    k = sequence_keys(*encode_batch(seqs, masks)[:3])
+    assert [int(x) for x in k] == [0x5102b3073b06032d, 0x169a62a6073b0122, 0x1758089107380121, 0x1,
+                                   0x75d6a059104c020f]
+    # position-independent: the same sequence keys alike wherever it sits
+    k2 = sequence_keys(*encode_batch(["ACGU", "GGGGAAAACCCC"])[:3])
+    assert int(k2[1]) == 0x6fe83a4314ea032d
+
+
 def test_param_map_linear_and_gradient_transpose():
     from contrafold_gpu_amd import params
     names, raw, pm = params.load_raw()
