@@ -572,10 +572,12 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
                 // would leave most CUs idle, and two-wave teams finish sooner
                 // (1,000 x 128 nt, logZ + MEA + Viterbi: 34.2k -> 40.0k folds/s;
                 // measured gains hold up to ~2,000 x 128 nt and ~500 x 200 nt,
-                // beyond which packing wins).
-                if (width <= 0 && !c->tuned && w == 64 &&
-                    (int64_t)(end - pos) * Lb * Lb <= (int64_t)83000 * c->n_cu)
-                    w = 128;
+                // beyond which packing wins); at half that size four-wave teams
+                // are faster still (100 x 200 nt, logZ + 100 samples: 13.2 ->
+                // 12.3 ms).
+                const int64_t work = (int64_t)(end - pos) * Lb * Lb;
+                if (width <= 0 && !c->tuned && w == 64 && work <= (int64_t)83000 * c->n_cu)
+                    w = (2 * work <= (int64_t)83000 * c->n_cu) ? 256 : 128;
                 b->buckets.push_back({pos, end, w, Lb});
             }
             pos = end;
