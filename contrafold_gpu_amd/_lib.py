@@ -47,6 +47,7 @@ SIGNATURES = [
     ("cf_batch_device_bytes", _c.c_int64, [_vp]),
     ("cf_batch_destroy", None, [_vp]),
     ("cf_fold_bytes", _c.c_int64, [_c.c_int, _c.c_uint, _c.c_int, _c.c_int]),
+    ("cf_copy_probe", _c.c_int, [_vp, _c.c_int64]),
     ("cf_server_create", _c.c_int, [_vp, _c.c_int64, _c.c_int, _c.c_int, _c.POINTER(_vp)]),
     ("cf_server_submit", _c.c_int64, [_vp, _c.c_char_p, _c.c_int, _u8p, _c.c_uint, _c.c_float, _c.c_int,
                                       _c.c_uint64, _c.c_double]),
@@ -62,6 +63,38 @@ SIGNATURES = [
 _lib = None
 
 
+def hip_runtimes():
+    """Distinct HIP runtime library files mapped into this process."""
+    found = set()
+    try:
+        with open("/proc/self/maps") as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) >= 6 and "libamdhip64" in os.path.basename(parts[-1]):
+                    found.add(os.path.realpath(parts[-1]))
+    except OSError:
+        pass
+    return found
+
+
+def check_single_runtime(what="the fold engine"):
+    """Raise if two HIP runtimes share this process.
+
+    The engine library links libamdhip64.so.7.  PyTorch's wheel bundles its
+    own copy whose soname is the same, but its libraries ask for it by the
+    file name libamdhip64.so, which the dynamic loader does not match against
+    an already loaded libamdhip64.so.7.  So importing torch *before* the
+    engine gives one runtime (the engine binds to torch's copy by soname),
+    while importing torch *after* the engine has started loads a second HIP
+    runtime with its own HSA runtime into the process: two runtimes then
+    drive one GPU, and GPU work of the one started second has crashed
+    (notably under rocprofv3 counter collection, which instruments both)."""
+    rts = hip_runtimes()
+    if len(rts) > 1:
+        raise ImportError(f"{what}: two HIP runtimes are loaded ({', '.join(sorted(rts))}); import torch "
+                          f"before contrafold_gpu_amd starts its engine, or not at all")
+
+
 def load_library(path=None):
     """Load and bind the shared library (raises OSError if it is not built)."""
     global _lib
@@ -71,6 +104,7 @@ def load_library(path=None):
     if not os.path.exists(p):
         raise OSError(f"{p} is not built; run `python -m contrafold_gpu_amd.build` (needs hipcc)")
     lib = ctypes.CDLL(p)
+    check_single_runtime()
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)
         fn.restype = res

@@ -9,13 +9,17 @@ weights, and maps the table gradient to the raw weights through the (linear)
 raw -> table map: d logZ / d theta = J^T E[feature counts].
 
 PyTorch is only the autograd plumbing here; all arithmetic is in the HIP
-library.
+library.  Import this module (or torch) before the engine first runs: torch
+brings its own HIP runtime, and loading it after the engine's would put two
+runtimes in one process (``_lib.check_single_runtime`` raises then).
 """
 import numpy as np
 import torch
 
 from . import params as _params
-from ._lib import CF_GRAD, CF_LOGZ
+from ._lib import CF_GRAD, CF_LOGZ, check_single_runtime
+
+check_single_runtime("contrafold_gpu_amd.autograd")
 
 
 class LogZFunction(torch.autograd.Function):

@@ -64,10 +64,14 @@ size_t measure(F f) {
     return align_up(c.off, 256);
 }
 
-int region_get(cf_ctx* c, Region& r, size_t n) {
+// `gen` counts pointer changes of the regions of one batch: a captured graph
+// holds the addresses it was recorded with and is replayed only while the
+// count is unchanged.
+int region_get(cf_ctx* c, Region& r, size_t n, uint64_t* gen = nullptr) {
     if (r.p && r.n >= n) return CF_OK;
     if (r.p) c->pool.put(r.p, r.n);
     r = Region();
+    if (gen) ++*gen;
     void* p = nullptr;
     size_t got = 0;
     hipError_t e = c->pool.get(&p, n, &got);
@@ -81,9 +85,11 @@ void region_put(cf_ctx* c, Region& r) {
     r = Region();
 }
 
-__global__ void cf_fill_f64(double* p, int64_t n, double v) {
+// Calibration copy for hardware traffic counters: each thread moves 16 bytes
+// per step, read once and written once.
+__global__ void cf_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+        dst[i] = src[i];
 }
 
 int mea_lds(int Lmax) { return (int)align_up((size_t)(Lmax + 2) * 4 + 2 * (Lmax + 2) + 16, 16) + 2 * (Lmax + 2) * 4 + 16; }
@@ -260,26 +266,33 @@ int launch_sample(cf_batch* b, uint64_t seed, hipStream_t st) {
     if (local)
         hipLaunchKernelGGL(cf_sample_kernel<true>, dim3(b->sgrid), dim3(CF_SAMP_WAVES * WAVE), lds, st, b->d_seqs,
                            b->d_order, b->n, b->n_samples, seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC,
-                           b->FM, b->FM1, b->FM2, b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff, b->Lmax);
+                           b->FM, b->FM1, b->FM2, b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff, b->Lmax,
+                           b->d_status);
     else
         hipLaunchKernelGGL(cf_sample_kernel<false>, dim3(b->sgrid), dim3(CF_SAMP_WAVES * WAVE), lds, st, b->d_seqs,
                            b->d_order, b->n, b->n_samples, seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC,
-                           b->FM, b->FM1, b->FM2, b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff, b->Lmax);
+                           b->FM, b->FM1, b->FM2, b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff, b->Lmax,
+                           b->d_status);
     CF_HIP(c, hipGetLastError(), "sample launch");
     return CF_OK;
 }
 
+// Per-sequence expected counts (one row each), then their weighted sum in
+// sequence order: the gradient is the same bit for bit whatever the buckets,
+// team widths or scheduling.
 int launch_grad(cf_batch* b, hipStream_t st) {
     cf_ctx* c = b->ctx;
-    hipLaunchKernelGGL(cf_fill_f64, dim3(4), dim3(256), 0, st, b->grad, (int64_t)CF_NTAB, 0.0);
     for (const auto& bk : b->buckets) {
         const int nl = bk.end - bk.begin;
         if (nl <= 0) continue;
         hipLaunchKernelGGL(cf_grad_kernel, dim3(nl), dim3(solo_threads(bk)), 0, st, b->d_seqs, b->d_order + bk.begin,
                            b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2, b->F5, b->FCo, b->FMo,
-                           b->FM1o, b->F5o, (b->d_gin + kMaxGamma), b->grad);
+                           b->FM1o, b->F5o, b->seqcnt);
         CF_HIP(c, hipGetLastError(), "grad launch");
     }
+    hipLaunchKernelGGL(cf_grad_reduce_kernel, dim3((CF_NTAB + 255) / 256), dim3(256), 0, st, b->seqcnt, b->n,
+                       (const float*)(b->d_gin + kMaxGamma), b->grad);
+    CF_HIP(c, hipGetLastError(), "grad reduce launch");
     return CF_OK;
 }
 
@@ -292,7 +305,7 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
             size_t need = measure([&](Carver& k) {
                 for (int t = 0; t < 5; ++t) k.take<float>(b->total_T);
             });
-            if ((rc = region_get(c, b->r_out, need))) return rc;
+            if ((rc = region_get(c, b->r_out, need, &b->buf_gen))) return rc;
             Carver k{(char*)b->r_out.p};
             b->FCo = k.take<float>(b->total_T);
             b->FMo = k.take<float>(b->total_T);
@@ -309,7 +322,7 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
                 k.take<int16_t>((int64_t)n_gamma * b->total_T);
                 k.take<uint8_t>((int64_t)n_gamma * b->total_chars);
             };
-            if ((rc = region_get(c, b->r_mea, measure(lay)))) return rc;
+            if ((rc = region_get(c, b->r_mea, measure(lay), &b->buf_gen))) return rc;
             Carver k{(char*)b->r_mea.p};
             b->SC = k.take<float>((int64_t)n_gamma * b->total_T);
             b->TB = k.take<int16_t>((int64_t)n_gamma * b->total_T);
@@ -324,7 +337,7 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
             k.take<double>(b->n);
             k.take<uint8_t>(b->total_chars);
         };
-        if ((rc = region_get(c, b->r_vit, measure(lay)))) return rc;
+        if ((rc = region_get(c, b->r_vit, measure(lay), &b->buf_gen))) return rc;
         Carver k{(char*)b->r_vit.p};
         b->VFC = k.take<double>(b->total_T);
         b->VFM = k.take<double>(b->total_T);
@@ -348,7 +361,7 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
                 k.take<uint8_t>((int64_t)ns_cap * b->total_chars);
                 k.take<uint32_t>((int64_t)g_cap * CF_SAMP_WAVES * cap);
             };
-            if ((rc = region_get(c, b->r_smp, measure(lay)))) return rc;
+            if ((rc = region_get(c, b->r_smp, measure(lay), &b->buf_gen))) return rc;
             Carver k{(char*)b->r_smp.p};
             b->samp_out = k.take<uint8_t>((int64_t)ns_cap * b->total_chars);
             b->stack = k.take<uint32_t>((int64_t)g_cap * CF_SAMP_WAVES * cap);
@@ -360,10 +373,12 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
     if ((flags & CF_GRAD) && !b->r_grad.p) {
         auto lay = [&](Carver& k) {
             k.take<double>(CF_NTAB);
+            k.take<double>((int64_t)b->n * CF_NTAB);
         };
-        if ((rc = region_get(c, b->r_grad, measure(lay)))) return rc;
+        if ((rc = region_get(c, b->r_grad, measure(lay), &b->buf_gen))) return rc;
         Carver k{(char*)b->r_grad.p};
         b->grad = k.take<double>(CF_NTAB);
+        b->seqcnt = k.take<double>((int64_t)b->n * CF_NTAB);
     }
     return CF_OK;
 }
@@ -525,6 +540,7 @@ size_t batch_staging_bytes(int n, int64_t C) {
         k.take<int64_t>(n);
         k.take<int64_t>(n);
         k.take<float>(kMaxGamma + n);
+        k.take<int>(4);
     });
 }
 
@@ -673,6 +689,7 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
     b->d_charoff = d.take<int64_t>(n);
     b->d_soff = d.take<int64_t>(n);
     b->d_gin = d.take<float>(kMaxGamma + n);
+    b->d_status = d.take<int>(4);
     for (auto& ev : b->ev) {
         if (hipEventCreate(&ev) != hipSuccess) {
             cf_batch_destroy(b);
@@ -711,16 +728,18 @@ int batch_enqueue(cf_batch* b, unsigned flags, const float* gammas, int n_gamma,
     }
     int rc = ensure_buffers(b, need, n_gamma, n_samples);
     if (rc) return rc;
-    b->n_gamma = n_gamma;
-    b->n_samples = n_samples;
+    // a stage that does not run keeps the results (and their counts) of its last run
+    if (need & CF_MEA) b->n_gamma = n_gamma;
+    if (need & CF_SAMPLE) b->n_samples = n_samples;
     if (b->params_version != c->params_version) b->done = 0;
     unsigned run = need;
     if (!(flags & CF_RECOMPUTE) && !(flags & CF_GRAPH)) {
         if (b->done & CF_LOGZ) run &= ~CF_LOGZ;
         if (b->done & CF_BPP) run &= ~CF_BPP;
     }
-    // per-run inputs: sample offsets, gammas, gradient weights (pinned -> device, async)
-    const size_t tail = sizeof(int64_t) * (size_t)b->n + 256 + sizeof(float) * (kMaxGamma + (size_t)b->n);
+    // per-run inputs: sample offsets, gammas, gradient weights and a cleared
+    // status word (pinned -> device, async; outside any captured graph)
+    const size_t tail = (size_t)((char*)(b->d_status + 4) - (char*)b->d_soff);
     if (staging_bytes < tail) return fail(c, CF_EINVAL, "staging buffer too small");
     {
         char* base = (char*)staging;
@@ -730,14 +749,15 @@ int batch_enqueue(cf_batch* b, unsigned flags, const float* gammas, int n_gamma,
         float* gw = (float*)(base + gap);
         for (int g = 0; g < kMaxGamma; ++g) gw[g] = g < n_gamma ? gammas[g] : 1.0f;
         for (int i = 0; i < b->n; ++i) gw[kMaxGamma + i] = weights ? weights[i] : 1.0f;
-        CF_HIP(c, hipMemcpyAsync(b->d_soff, staging, gap + sizeof(float) * (kMaxGamma + b->n), hipMemcpyHostToDevice,
-                                 b->stream),
-               "per-run inputs");
+        std::memset(base + ((char*)b->d_status - (char*)b->d_soff), 0, 4 * sizeof(int));
+        CF_HIP(c, hipMemcpyAsync(b->d_soff, staging, tail, hipMemcpyHostToDevice, b->stream), "per-run inputs");
     }
     CF_HIP(c, hipEventRecord(b->ev[kNumKernels + 1], b->stream), "event");
     if (flags & CF_GRAPH) {
+        // the graph replays the launches it captured, with the buffer addresses
+        // of that moment: any reallocation since then forces a new capture
         const bool reuse = b->gexec && b->g_flags == run && b->g_ngamma == n_gamma && b->g_nsamp == n_samples &&
-                           b->g_seed == seed;
+                           b->g_seed == seed && b->g_gen == b->buf_gen;
         if (!reuse) {
             if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
             if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
@@ -755,6 +775,7 @@ int batch_enqueue(cf_batch* b, unsigned flags, const float* gammas, int n_gamma,
             b->g_ngamma = n_gamma;
             b->g_nsamp = n_samples;
             b->g_seed = seed;
+            b->g_gen = b->buf_gen;
         }
         CF_HIP(c, hipGraphLaunch(b->gexec, b->stream), "graph launch");
     } else {
@@ -850,6 +871,27 @@ int cf_set_params(cf_ctx* c, const double* tables) {
     return CF_OK;
 }
 
+int cf_copy_probe(cf_ctx* c, int64_t bytes) {
+    if (!c || bytes <= 0 || bytes % 16) return fail(c, CF_EINVAL, "copy probe: bytes must be a positive multiple of 16");
+    CF_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    void *src = nullptr, *dst = nullptr;
+    CF_HIP(c, hipMalloc(&src, (size_t)bytes), "hipMalloc probe");
+    hipError_t e = hipMalloc(&dst, (size_t)bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(src);
+        return hipfail(c, e, "hipMalloc probe");
+    }
+    const int64_t n = bytes / 16;
+    hipLaunchKernelGGL(cf_copy_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65536)), dim3(256), 0,
+                       c->stream, (const float4*)src, (float4*)dst, n);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e != hipSuccess) return hipfail(c, e, "copy probe");
+    return CF_OK;
+}
+
 int cf_set_tuning(cf_ctx* c, const int* edges, const int* threads, int nb) {
     if (!c || !edges || !threads || nb <= 0) return fail(c, CF_EINVAL, "bad tuning arguments");
     std::vector<int> e(edges, edges + nb), t(threads, threads + nb);
@@ -906,6 +948,14 @@ int cf_batch_run(cf_batch* b, unsigned flags, const float* gammas, int n_gamma, 
         return rc;
     }
     CF_HIP(c, hipStreamSynchronize(b->stream), "fold run");
+    if (ran & CF_SAMPLE) {
+        int st = 0;
+        CF_HIP(c, hipMemcpy(&st, b->d_status, sizeof(int), hipMemcpyDeviceToHost), "read status");
+        if (st & 1) {
+            b->done &= ~(unsigned)CF_SAMPLE;
+            return fail(c, CF_EDEVICE, "sampler traceback stack overflow (samples discarded)");
+        }
+    }
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, b->ev[cfi::kNumKernels + 1], b->ev[cfi::kNumKernels + 2]) == hipSuccess)
         b->last_ms = ms;

@@ -102,6 +102,7 @@ struct cf_batch {
     int64_t* d_charoff = nullptr;
     int64_t* d_soff = nullptr;
     float* d_gin = nullptr;     // per-run small inputs: gammas (64) then grad weights (n)
+    int* d_status = nullptr;    // per-run kernel status bits (1: sampler stack overflow), zeroed by each run
     float *FC = nullptr, *FM = nullptr, *FM1 = nullptr, *FM2 = nullptr, *FCX = nullptr, *F5 = nullptr,
           *F5o = nullptr, *logz = nullptr;
     // outside
@@ -125,6 +126,7 @@ struct cf_batch {
     // gradient
     Region r_grad;
     double* grad = nullptr;
+    double* seqcnt = nullptr;  // per-sequence expected counts, n x 2577
     unsigned done = 0;
     uint64_t params_version = 0;
     // stage marks 0..8, run start, run end, and the start / end of a sampler
@@ -143,6 +145,7 @@ struct cf_batch {
     unsigned g_flags = 0;
     int g_ngamma = -1, g_nsamp = -1;
     uint64_t g_seed = 0;
+    uint64_t buf_gen = 0, g_gen = 0;  // region reallocations; the count the graph was captured at
     int64_t device_bytes() const {
         return (int64_t)(r_in.n + r_inside.n + r_out.n + r_mea.n + r_vit.n + r_smp.n + r_grad.n);
     }

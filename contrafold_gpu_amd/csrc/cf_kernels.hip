@@ -1957,7 +1957,7 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
     const CfParams<float>* __restrict__ P, const float* __restrict__ FC, const float* __restrict__ FM,
     const float* __restrict__ FM1, const float* __restrict__ FM2, const float* __restrict__ F5,
     uint32_t* __restrict__ stack, int stack_cap, uint8_t* __restrict__ out, const int64_t* __restrict__ out_off,
-    int lmax) {
+    int lmax, int* __restrict__ status) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ CfParams<float> sp;
     __shared__ uint8_t wl1[496], wl2[496];  // single-branch window order: l1 ascending, l2 ascending
@@ -2035,12 +2035,18 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
         rng.ctr = 0;
         rng.have = 0;
         // the stack is wave-uniform: every lane stores the same entry, and a
-        // popped entry is read into scalar registers
+        // popped entry is read into scalar registers.  Depth bound: a popped
+        // state is replaced by states over disjoint, non-empty sub-intervals of
+        // its own interval (only the exterior state F5(0) is empty), so at most
+        // L + 1 entries are ever held; stack_cap = 2 Lmax + 8.  A push beyond
+        // the cap is still reported (status bit 1) rather than dropped silently.
         int sp_ = 0;
+        bool overflow = false;
         auto push = [&](int typ, int i, int j) {
             const uint32_t e = (uint32_t)typ | ((uint32_t)i << 2) | ((uint32_t)j << 17);
             if (sp_ < CF_SAMP_STK) lstk[sp_++] = e;
             else if (sp_ < stack_cap) st[sp_++] = e;
+            else overflow = true;
         };
         // multiloop split of FM2[i][j]: k with weight FM1[i][k] + FM[k][j];
         // the next chunk's cells are requested before the current one is scanned
@@ -2239,6 +2245,7 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
                 }
             }
         }
+        if (overflow && lane == 0) atomicOr(status, 1);
     }
 }
 
@@ -2246,23 +2253,33 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
 // Expected feature counts: every decomposition event's probability
 // exp(outside + event + inside - Z) is added to each parameter-table entry the
 // event's score uses.  The counts equal d logZ / d table (the outside pass is
-// the adjoint of the inside pass).  Accumulated per sequence in LDS, then
-// weighted into a global FP64 vector.
-__device__ __forceinline__ void cf_cnt(float* cnt, int idx, float p) { atomicAdd(&cnt[idx], p); }
+// the adjoint of the inside pass).
+//
+// Determinism: a sequence's counts are accumulated in LDS as 64-bit fixed
+// point (units of 2^-40): integer addition is associative, so the result does
+// not depend on which thread adds an event first.  Each workgroup writes its
+// sequence's counts (FP64) to its own row of `seqcnt`; cf_grad_reduce_kernel
+// then sums the rows in sequence order.  A probability is at most ~1 and a
+// sequence's count of one entry at most a few times L, so 2^40 units leave
+// more than 2^8 of headroom below 2^63 at L = 32,000; the rounding step
+// (2^-41 per event) is far below the FP32 probabilities' own precision.
+#define CF_GRAD_FIX 1099511627776.0f  // 2^40
+__device__ __forceinline__ void cf_cnt(unsigned long long* cnt, int idx, float p) {
+    atomicAdd(&cnt[idx], (unsigned long long)__float2ull_rn(p * CF_GRAD_FIX));
+}
 
 __global__ void __launch_bounds__(256) cf_grad_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ order, const uint8_t* __restrict__ codes,
     const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
     const float* __restrict__ FM, const float* __restrict__ FM1, const float* __restrict__ FM2,
     const float* __restrict__ F5, const float* __restrict__ FCo, const float* __restrict__ FMo,
-    const float* __restrict__ FM1o, const float* __restrict__ F5o, const float* __restrict__ weights,
-    double* __restrict__ grad) {
-    __shared__ float cnt[CF_NTAB];
+    const float* __restrict__ FM1o, const float* __restrict__ F5o, double* __restrict__ seqcnt) {
+    __shared__ unsigned long long cnt[CF_NTAB];
     const int bseq = order[blockIdx.x];
     const CfSeq sd = seqs[bseq];
     const int L = sd.L;
     const int tid = threadIdx.x, NT = blockDim.x;
-    for (int t = tid; t < CF_NTAB; t += NT) cnt[t] = 0.0f;
+    for (int t = tid; t < CF_NTAB; t += NT) cnt[t] = 0ull;
     __syncthreads();
     const uint8_t* __restrict__ s = codes + sd.coff;
     const uint8_t* __restrict__ fo = masks ? masks + sd.coff : nullptr;
@@ -2392,9 +2409,20 @@ __global__ void __launch_bounds__(256) cf_grad_kernel(
         }
     }
     __syncthreads();
-    const double w = weights ? (double)weights[bseq] : 1.0;
-    for (int t = tid; t < CF_NTAB; t += NT)
-        if (cnt[t] != 0.0f) atomicAdd(&grad[t], w * (double)cnt[t]);
+    double* __restrict__ row = seqcnt + (int64_t)bseq * CF_NTAB;
+    for (int t = tid; t < CF_NTAB; t += NT) row[t] = (double)cnt[t] * (1.0 / (double)CF_GRAD_FIX);
+}
+
+// grad[t] = sum over sequences b = 0, 1, ..., n-1 (in that order) of
+// w_b * seqcnt[b][t]: one thread per table entry, so the sum's order is fixed.
+__global__ void __launch_bounds__(256) cf_grad_reduce_kernel(const double* __restrict__ seqcnt, int n,
+                                                             const float* __restrict__ weights,
+                                                             double* __restrict__ grad) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= CF_NTAB) return;
+    double acc = 0.0;
+    for (int b = 0; b < n; ++b) acc += (weights ? (double)weights[b] : 1.0) * seqcnt[(int64_t)b * CF_NTAB + t];
+    grad[t] = acc;
 }
 
 // ------------------------------------------------------------ explicit inits

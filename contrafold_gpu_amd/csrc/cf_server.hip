@@ -34,6 +34,7 @@ struct Req {
     uint64_t seed = 0;
     double arrival_us = 0.0, launch_us = 0.0, done_us = 0.0;
     int state = 0;  // 0 pending, 1 in flight, 2 done, 3 failed
+    int fail_code = CF_ENOMEM;  // status a failed request reports
     int64_t bytes = 0;
     std::vector<char> result;
     float logz = 0.0f;
@@ -105,10 +106,14 @@ int retire(cf_server* s, Slot& sl) {
     s->run_us += t - sl.launched_us;
     const float* lz = (const float*)sl.out;
     const char* pay = (const char*)sl.out + sizeof(float) * (size_t)b->n;
+    int status = 0;
+    if (sl.kind == CF_SAMPLE)
+        std::memcpy(&status, pay + (size_t)b->n_samples * (size_t)b->total_chars, sizeof(int));
     int64_t toff = 0;
     for (size_t k = 0; k < sl.reqs.size(); ++k) {
         Req& r = s->reqs[sl.reqs[k]];
-        r.state = 2;
+        r.state = status ? 3 : 2;
+        if (status) r.fail_code = CF_EDEVICE;
         r.done_us = t;
         r.logz = lz[k];
         const int L = b->L[k];
@@ -262,7 +267,9 @@ int launch(cf_server* s, Slot& sl) {
         if (kind == CF_MEA) src = b->mea_out, pay = (size_t)b->total_chars;
         else if (kind == CF_SAMPLE) src = b->samp_out, pay = (size_t)b->n_samples * b->total_chars;
         else if (kind == CF_BPP) src = b->POST, pay = sizeof(float) * (size_t)b->total_T;
-        const size_t need = sizeof(float) * (size_t)n + pay;
+        // a sampling batch also brings back its kernel status word
+        const size_t stat = kind == CF_SAMPLE ? sizeof(int) : 0;
+        const size_t need = sizeof(float) * (size_t)n + pay + stat;
         if (!grow_pinned(s, &sl.out, &sl.out_bytes, need)) rc = cfi::fail(c, CF_ENOMEM, "pinned result buffer");
         if (!rc && hipMemcpyAsync(sl.out, b->logz, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, sl.stream) !=
                        hipSuccess)
@@ -270,6 +277,10 @@ int launch(cf_server* s, Slot& sl) {
         if (!rc && pay &&
             hipMemcpyAsync((char*)sl.out + sizeof(float) * (size_t)n, src, pay, hipMemcpyDeviceToHost, sl.stream) !=
                 hipSuccess)
+            rc = cfi::fail(c, CF_EDEVICE, "result copy");
+        if (!rc && stat &&
+            hipMemcpyAsync((char*)sl.out + sizeof(float) * (size_t)n + pay, b->d_status, stat, hipMemcpyDeviceToHost,
+                           sl.stream) != hipSuccess)
             rc = cfi::fail(c, CF_EDEVICE, "result copy");
     }
     if (!rc && hipEventRecord(sl.done, sl.stream) != hipSuccess) rc = cfi::fail(c, CF_EDEVICE, "event record");
@@ -369,8 +380,11 @@ int cf_server_result(cf_server* s, int64_t id, char* structure, float* logz, dou
     if (it == s->reqs.end()) return cfi::fail(s->ctx, CF_EINVAL, "unknown request id");
     Req& r = it->second;
     if (r.state == 3) {
+        const int code = r.fail_code;
         s->reqs.erase(it);
-        return cfi::fail(s->ctx, CF_ENOMEM, "request failed (does not fit the HBM budget or device error)");
+        return cfi::fail(s->ctx, code,
+                         code == CF_ENOMEM ? "request failed (does not fit the HBM budget or device error)"
+                                           : "request failed (sampler traceback stack overflow)");
     }
     if (r.state != 2) return cfi::fail(s->ctx, CF_ESTATE, "request not complete");
     if (structure && !r.result.empty()) std::memcpy(structure, r.result.data(), r.result.size());

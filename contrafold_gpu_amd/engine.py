@@ -4,10 +4,16 @@ An ``Engine`` owns one C-ABI context (one HIP stream + device parameter
 tables).  A ``Batch`` owns the device-resident DP tables of its sequences: a
 logZ run leaves the inside tables on the GPU, and a later bpp / MEA / sample /
 gradient run on the same batch reuses them.
+
+Threads: the C context is not re-entrant (its pinned staging buffer, device
+pool and error string are shared by every call on it), and ctypes releases
+the GIL during a call.  Every call into the context therefore holds the
+engine's lock; the drop-in functions hold it across a whole batch's life.
 """
 import ctypes
 import hashlib
 import os
+import threading
 import zlib
 
 import numpy as np
@@ -93,11 +99,13 @@ class Engine:
             raise RuntimeError(f"cf_create(device={device}) failed with status {rc}: no usable HIP device")
         self.ctx = ctx
         self._fp = None
+        self.lock = threading.RLock()
 
     def close(self):
-        if self.ctx:
-            self.lib.cf_destroy(self.ctx)
-            self.ctx = None
+        with self.lock:
+            if self.ctx:
+                self.lib.cf_destroy(self.ctx)
+                self.ctx = None
 
     def __del__(self):
         try:
@@ -111,15 +119,17 @@ class Engine:
         v = P if isinstance(P, np.ndarray) else _params.flatten(P)
         v = np.ascontiguousarray(v, np.float64)
         fp = hashlib.blake2b(v.tobytes(), digest_size=16).digest()
-        if fp != self._fp:
-            check(self.lib.cf_set_params(self.ctx, v.ctypes.data_as(_c.POINTER(_c.c_double))), self.ctx,
-                  "cf_set_params")
-            self._fp = fp
+        with self.lock:
+            if fp != self._fp:
+                check(self.lib.cf_set_params(self.ctx, v.ctypes.data_as(_c.POINTER(_c.c_double))), self.ctx,
+                      "cf_set_params")
+                self._fp = fp
 
     def set_tuning(self, edges, threads):
         e = (_c.c_int * len(edges))(*edges)
         t = (_c.c_int * len(threads))(*threads)
-        check(self.lib.cf_set_tuning(self.ctx, e, t, len(edges)), self.ctx, "cf_set_tuning")
+        with self.lock:
+            check(self.lib.cf_set_tuning(self.ctx, e, t, len(edges)), self.ctx, "cf_set_tuning")
 
     def batch(self, seqs, masks=None, keys=None, sampling=True):
         return Batch(self, seqs, masks, keys, sampling)
@@ -146,16 +156,19 @@ class Batch:
             keys = np.ascontiguousarray(keys, np.uint64)
             kptr = keys.ctypes.data_as(_c.POINTER(_c.c_uint64))
         h = _c.c_void_p()
-        check(self.lib.cf_batch_create(engine.ctx, self.n, buf, offs.ctypes.data_as(_c.POINTER(_c.c_int64)),
-                                       mbuf.ctypes.data if mbuf is not None else None, kptr, _c.byref(h)),
-              engine.ctx, "cf_batch_create")
+        self.lock = engine.lock
+        with self.lock:
+            check(self.lib.cf_batch_create(engine.ctx, self.n, buf, offs.ctypes.data_as(_c.POINTER(_c.c_int64)),
+                                           mbuf.ctypes.data if mbuf is not None else None, kptr, _c.byref(h)),
+                  engine.ctx, "cf_batch_create")
         self.h = h
         self.n_gamma = 0
         self.n_samples = 0
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.cf_batch_destroy(self.h)
+            with self.lock:
+                self.lib.cf_batch_destroy(self.h)
             self.h = None
 
     def __enter__(self):
@@ -178,15 +191,21 @@ class Batch:
             if w.size != self.n:
                 raise ValueError(f"{w.size} weights for {self.n} sequences")
         f = int(flags) | (CF_RECOMPUTE if recompute else 0) | (CF_TIMING if timing else 0)
-        check(self.lib.cf_batch_run(self.h, f, g.ctypes.data_as(_c.POINTER(_c.c_float)) if g.size else None,
-                                    int(g.size), int(n_samples), int(seed) & 0xFFFFFFFFFFFFFFFF,
-                                    w.ctypes.data_as(_c.POINTER(_c.c_float)) if w is not None else None),
-              self.engine.ctx, "cf_batch_run")
+        with self.lock:
+            check(self.lib.cf_batch_run(self.h, f, g.ctypes.data_as(_c.POINTER(_c.c_float)) if g.size else None,
+                                        int(g.size), int(n_samples), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                        w.ctypes.data_as(_c.POINTER(_c.c_float)) if w is not None else None),
+                  self.engine.ctx, "cf_batch_run")
         if flags & CF_MEA:
             self.n_gamma = int(g.size)
         if flags & CF_SAMPLE:
             self.n_samples = int(n_samples)
         return self
+
+    def _call(self, what, fn, *args):
+        """One read-back entry point on this batch, under the engine's lock."""
+        with self.lock:
+            return check(fn(self.h, *args), self.engine.ctx, what)
 
     def last_ms(self):
         return float(self.lib.cf_batch_last_ms(self.h))
@@ -194,7 +213,7 @@ class Batch:
     def kernel_ms(self):
         k = self.lib.cf_kernel_count()
         out = (_c.c_float * k)()
-        check(self.lib.cf_batch_kernel_ms(self.h, out), self.engine.ctx)
+        self._call("cf_batch_kernel_ms", self.lib.cf_batch_kernel_ms, out)
         return {self.lib.cf_kernel_name(i).decode(): float(out[i]) for i in range(k)}
 
     def device_bytes(self):
@@ -202,15 +221,13 @@ class Batch:
 
     def logz(self):
         out = np.empty(self.n, np.float32)
-        check(self.lib.cf_batch_logz(self.h, out.ctypes.data_as(_c.POINTER(_c.c_float))), self.engine.ctx,
-              "cf_batch_logz")
+        self._call("cf_batch_logz", self.lib.cf_batch_logz, out.ctypes.data_as(_c.POINTER(_c.c_float)))
         return out
 
     def bpp(self, index):
         L = int(self.lens[index])
         out = np.zeros((L, L), np.float32)
-        check(self.lib.cf_batch_bpp(self.h, int(index), out.ctypes.data_as(_c.POINTER(_c.c_float))),
-              self.engine.ctx, "cf_batch_bpp")
+        self._call("cf_batch_bpp", self.lib.cf_batch_bpp, int(index), out.ctypes.data_as(_c.POINTER(_c.c_float)))
         return out
 
     def _split(self, raw, per=None):
@@ -227,20 +244,19 @@ class Batch:
 
     def mea(self, gamma_index=0):
         buf = _c.create_string_buffer(int(self.offs[-1]) + 1)
-        check(self.lib.cf_batch_mea(self.h, int(gamma_index), buf), self.engine.ctx, "cf_batch_mea")
+        self._call("cf_batch_mea", self.lib.cf_batch_mea, int(gamma_index), buf)
         return self._split(buf.raw[:int(self.offs[-1])])
 
     def mfe(self):
         buf = _c.create_string_buffer(int(self.offs[-1]) + 1)
         sc = np.empty(self.n, np.float64)
-        check(self.lib.cf_batch_mfe(self.h, buf, sc.ctypes.data_as(_c.POINTER(_c.c_double))), self.engine.ctx,
-              "cf_batch_mfe")
+        self._call("cf_batch_mfe", self.lib.cf_batch_mfe, buf, sc.ctypes.data_as(_c.POINTER(_c.c_double)))
         return self._split(buf.raw[:int(self.offs[-1])]), sc
 
     def samples_raw(self):
         nb = int(self.offs[-1]) * self.n_samples
         buf = _c.create_string_buffer(nb + 1)
-        check(self.lib.cf_batch_samples(self.h, buf), self.engine.ctx, "cf_batch_samples")
+        self._call("cf_batch_samples", self.lib.cf_batch_samples, buf)
         return buf.raw[:nb]
 
     def samples(self):
@@ -255,12 +271,12 @@ class Batch:
         L = int(self.lens[index])
         if which >= 9:
             out = np.empty(L + 2, np.float32)
-            check(self.lib.cf_batch_table(self.h, int(index), which, out.ctypes.data_as(_c.POINTER(_c.c_float))),
-                  self.engine.ctx, "cf_batch_table")
+            self._call("cf_batch_table", self.lib.cf_batch_table, int(index), which,
+                       out.ctypes.data_as(_c.POINTER(_c.c_float)))
             return out
         tri = np.empty((L + 1) * (L + 2) // 2, np.float32)
-        check(self.lib.cf_batch_table(self.h, int(index), which, tri.ctypes.data_as(_c.POINTER(_c.c_float))),
-              self.engine.ctx, "cf_batch_table")
+        self._call("cf_batch_table", self.lib.cf_batch_table, int(index), which,
+                   tri.ctypes.data_as(_c.POINTER(_c.c_float)))
         dense = np.full((L + 2, L + 2), np.nan, np.float32)
         pos = 0
         for d in range(L + 1):
@@ -271,14 +287,12 @@ class Batch:
 
     def logz64(self):
         out = np.empty(self.n, np.float64)
-        check(self.lib.cf_batch_logz64(self.h, out.ctypes.data_as(_c.POINTER(_c.c_double))), self.engine.ctx,
-              "cf_batch_logz64")
+        self._call("cf_batch_logz64", self.lib.cf_batch_logz64, out.ctypes.data_as(_c.POINTER(_c.c_double)))
         return out
 
     def grad(self):
         out = np.empty(_params.N_TABLE, np.float64)
-        check(self.lib.cf_batch_grad(self.h, out.ctypes.data_as(_c.POINTER(_c.c_double))), self.engine.ctx,
-              "cf_batch_grad")
+        self._call("cf_batch_grad", self.lib.cf_batch_grad, out.ctypes.data_as(_c.POINTER(_c.c_double)))
         return out
 
 

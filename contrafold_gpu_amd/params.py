@@ -244,7 +244,16 @@ def load_raw(path=None):
 
 
 def write_raw(path, names, values):
-    """Write a raw vector back in the ``name value`` format."""
+    """Write a raw vector back in the ``name value`` format, one weight per
+    line in the given order.  Values are written with Python's shortest
+    round-trip representation, so reading the file back gives the same FP64
+    weights bit for bit (trained weights survive a save and a load)."""
+    names = list(names)
+    values = np.asarray(values, np.float64).reshape(-1)
+    if len(names) != values.size:
+        raise ValueError(f"{len(names)} names for {values.size} values")
+    if not np.isfinite(values).all():
+        raise ValueError("non-finite weight")
     with open(path, "w") as fh:
-        for n, v in zip(names, values):
-            fh.write(f"{n} {float(v):.10g}\n")
+        for n, v in zip(names, values.tolist()):
+            fh.write(f"{n} {v!r}\n")

@@ -45,14 +45,18 @@ class Server:
         self.engine = engine
         self.lib = engine.lib
         h = _c.c_void_p()
-        check(self.lib.cf_server_create(engine.ctx, int(hbm_budget), int(max_batch), int(n_streams), _c.byref(h)),
-              engine.ctx, "cf_server_create")
+        # the server drives the engine's context: its calls hold the engine's lock
+        self.lock = engine.lock
+        with self.lock:
+            check(self.lib.cf_server_create(engine.ctx, int(hbm_budget), int(max_batch), int(n_streams),
+                                            _c.byref(h)), engine.ctx, "cf_server_create")
         self.h = h
         self.meta = {}
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.cf_server_destroy(self.h)
+            with self.lock:
+                self.lib.cf_server_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -71,17 +75,19 @@ class Server:
                 raise ValueError(f"mask has {arr.size} entries for a sequence of length {L}")
             m = np.ascontiguousarray(arr[:L] != 0, np.uint8)
         key = request_key(seq, m, seed)
-        rid = self.lib.cf_server_submit(self.h, seq.encode("ascii", "replace"), L,
-                                        m.ctypes.data if m is not None else None, k, _c.c_float(gamma),
-                                        int(n_samples) if k == CF_SAMPLE else 0, key, 0.0)
-        check(int(rid), self.engine.ctx, "cf_server_submit")
+        with self.lock:
+            rid = self.lib.cf_server_submit(self.h, seq.encode("ascii", "replace"), L,
+                                            m.ctypes.data if m is not None else None, k, _c.c_float(gamma),
+                                            int(n_samples) if k == CF_SAMPLE else 0, key, 0.0)
+            check(int(rid), self.engine.ctx, "cf_server_submit")
         self.meta[int(rid)] = (kind, L, int(n_samples) if k == CF_SAMPLE else 0)
         return int(rid)
 
     def poll(self, cap=4096):
         ids = (_c.c_int64 * cap)()
-        n = self.lib.cf_server_poll(self.h, ids, cap)
-        check(n, self.engine.ctx, "cf_server_poll")
+        with self.lock:
+            n = self.lib.cf_server_poll(self.h, ids, cap)
+            check(n, self.engine.ctx, "cf_server_poll")
         return [int(ids[i]) for i in range(n)]
 
     def pending(self):
@@ -92,8 +98,9 @@ class Server:
     def stats(self):
         """Scheduler counters since creation (cf_server_stats)."""
         out = np.zeros(len(self.STATS), np.float64)
-        n = self.lib.cf_server_stats(self.h, out.ctypes.data_as(_c.POINTER(_c.c_double)), len(out))
-        check(min(n, 0), self.engine.ctx, "cf_server_stats")
+        with self.lock:
+            n = self.lib.cf_server_stats(self.h, out.ctypes.data_as(_c.POINTER(_c.c_double)), len(out))
+            check(min(n, 0), self.engine.ctx, "cf_server_stats")
         d = dict(zip(self.STATS, out.tolist()))
         for k in ("host_launch_ms", "host_retire_ms", "batch_residence_ms"):
             d[k] /= 1e3
@@ -105,8 +112,9 @@ class Server:
         buf = _c.create_string_buffer(max(size, 1))
         lz = _c.c_float()
         lat = _c.c_double()
-        check(self.lib.cf_server_result(self.h, rid, buf, _c.byref(lz), _c.byref(lat)), self.engine.ctx,
-              "cf_server_result")
+        with self.lock:
+            check(self.lib.cf_server_result(self.h, rid, buf, _c.byref(lz), _c.byref(lat)), self.engine.ctx,
+                  "cf_server_result")
         out = {"logz": float(lz.value), "latency_ms": lat.value / 1e3, "kind": kind}
         raw = buf.raw[:size]
         if kind == "mea":
@@ -144,15 +152,16 @@ class Server:
         lat = np.zeros(n, np.float64)
         wait = np.zeros(n, np.float64)
         wall = _c.c_double()
-        check(self.lib.cf_server_replay(self.h, n, buf, offs.ctypes.data_as(_c.POINTER(_c.c_int64)),
-                                        mbuf.ctypes.data if mbuf is not None else None,
-                                        kinds.ctypes.data_as(_c.POINTER(_c.c_uint)),
-                                        gammas.ctypes.data_as(_c.POINTER(_c.c_float)),
-                                        ns.ctypes.data_as(_c.POINTER(_c.c_int)),
-                                        arr.ctypes.data_as(_c.POINTER(_c.c_double)), int(seed),
-                                        lat.ctypes.data_as(_c.POINTER(_c.c_double)),
-                                        wait.ctypes.data_as(_c.POINTER(_c.c_double)), _c.byref(wall)),
-              self.engine.ctx, "cf_server_replay")
+        with self.lock:
+            check(self.lib.cf_server_replay(self.h, n, buf, offs.ctypes.data_as(_c.POINTER(_c.c_int64)),
+                                            mbuf.ctypes.data if mbuf is not None else None,
+                                            kinds.ctypes.data_as(_c.POINTER(_c.c_uint)),
+                                            gammas.ctypes.data_as(_c.POINTER(_c.c_float)),
+                                            ns.ctypes.data_as(_c.POINTER(_c.c_int)),
+                                            arr.ctypes.data_as(_c.POINTER(_c.c_double)), int(seed),
+                                            lat.ctypes.data_as(_c.POINTER(_c.c_double)),
+                                            wait.ctypes.data_as(_c.POINTER(_c.c_double)), _c.byref(wall)),
+                  self.engine.ctx, "cf_server_replay")
         if with_wait:
             return lat / 1e3, wall.value, wait / 1e3
         return lat / 1e3, wall.value
@@ -201,12 +210,18 @@ def plain_throughput(engine, trace, repeats=2):
     return len(trace["seqs"]) / dt
 
 
-def benchmark(engine, P, n=12000, fracs=(0.25, 0.5, 0.7, 0.85, 1.0), seed=5, log=print, n_streams=16, plain=None):
+def benchmark(engine, P, n=12000, fracs=(0.25, 0.5, 0.7, 0.85, 1.0), seed=5, log=print, n_streams=16, plain=None,
+              seconds=None, n_plain=None):
     """Serving workload: Poisson arrivals at fractions of the plain batched
-    capacity of the same mix; p50/p99 latency and served folds/s per rate."""
+    capacity of the same mix; p50/p99 latency and served folds/s per rate.
+
+    ``seconds``: if given, each rate's trace spans that many seconds of
+    arrivals (n = rate x seconds requests) instead of ``n`` requests, so every
+    rate reaches a steady state.  ``n_plain``: requests of the trace whose
+    plain batched throughput is the reference (default ``n``)."""
     from . import workloads
     engine.set_params(P)
-    base = workloads.serving_trace(n, 1.0, seed=seed)
+    base = workloads.serving_trace(n_plain or n, 1.0, seed=seed)
     if plain is None:
         plain = plain_throughput(engine, base)
     log(f"[serve] plain batched throughput of the mix: {plain:.0f} folds/s")
@@ -215,12 +230,15 @@ def benchmark(engine, P, n=12000, fracs=(0.25, 0.5, 0.7, 0.85, 1.0), seed=5, log
     try:
         for f in fracs:
             qps = f * plain
-            tr = workloads.serving_trace(n, qps, seed=seed)
+            n_f = int(max(500, qps * seconds)) if seconds else n
+            tr = workloads.serving_trace(n_f, qps, seed=seed)
             lat, wall, wait = srv.replay(tr, with_wait=True)
             ok = lat >= 0
             sustained = steady_rate(tr["arrival_s"], lat)
-            row = {"offered_frac": f, "offered_qps": round(qps, 1), "served_folds_per_s": round(n / wall, 1),
-                   "served_vs_plain": round(n / wall / plain, 4),
+            row = {"offered_frac": f, "offered_qps": round(qps, 1), "requests": n_f,
+                   "arrival_span_s": round(float(tr["arrival_s"][-1]), 2), "wall_s": round(wall, 2),
+                   "served_folds_per_s": round(n_f / wall, 1),
+                   "served_vs_plain": round(n_f / wall / plain, 4),
                    "sustained_folds_per_s": round(sustained, 1), "sustained_vs_plain": round(sustained / plain, 4),
                    "p50_ms": round(float(np.percentile(lat[ok], 50)), 3),
                    "p99_ms": round(float(np.percentile(lat[ok], 99)), 3), "failed": int((~ok).sum()),
@@ -240,7 +258,9 @@ def benchmark(engine, P, n=12000, fracs=(0.25, 0.5, 0.7, 0.85, 1.0), seed=5, log
         srv.close()
     best = [r for r in rows if r["p99_ms"] <= 50.0]
     head = max(best, key=lambda r: r["sustained_folds_per_s"]) if best else None
-    return {"n_streams": n_streams, "workload": f"{n} requests, log-normal lengths (median 150, sigma 0.6, 20-1000 nt), 30% masked "
+    span = f"{seconds:g} s of arrivals per rate" if seconds else f"{n} requests per rate"
+    return {"n_streams": n_streams,
+            "workload": f"{span}, log-normal lengths (median 150, sigma 0.6, 20-1000 nt), 30% masked "
                         f"(p=0.15), mix 70% MEA / 20% sample x10 / 10% bpp, Poisson arrivals",
             "plain_folds_per_s": round(plain, 1), "rates": rows,
             "best_at_p99_le_50ms": head}

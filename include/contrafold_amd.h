@@ -119,6 +119,13 @@ void cf_batch_destroy(cf_batch *b);
 /* Device bytes a fold of length L needs for the given flags (admission). */
 int64_t cf_fold_bytes(int L, unsigned flags, int n_gamma, int n_samples);
 
+/* Diagnostic, no counterpart in the reference: one device-to-device copy
+ * kernel over `bytes` (a positive multiple of 16; read once, written once) on
+ * the context stream, synchronous.  It is the known-size reference against
+ * which hardware traffic counters (rocprofv3 FETCH_SIZE / WRITE_SIZE) are
+ * scaled when the fold kernels' HBM bytes are measured. */
+int cf_copy_probe(cf_ctx *ctx, int64_t bytes);
+
 /* ---------------------------------------------------------------- serving
  * Continuous batching (no counterpart in the reference, which has no server).
  * Requests are admitted while their O(L^2) table bytes fit the HBM budget;

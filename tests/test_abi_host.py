@@ -256,3 +256,50 @@ def test_autotune_with_a_fake_engine(tmp_path, monkeypatch):
     fe = FakeEngine()
     big = autotune.tune(fe, ["ACGU" * 30] * 60000, sample=1500, repeats=1, use_cache=False)
     assert big["ms"] == 5.0 and set(fe.sizes) == {8000}  # capped at 8,000
+
+
+def test_one_hip_runtime_per_process():
+    """torch bundles its own HIP runtime; loading the engine first and torch
+    after would map two runtimes into one process.  Importing torch first
+    binds the engine to torch's copy; the other order makes the autograd
+    module refuse to import (no GPU call is made: loading only)."""
+    import subprocess
+    import sys
+    pytest.importorskip("torch")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    first = ("import sys; sys.path.insert(0, %r); import torch; from contrafold_gpu_amd import _lib; "
+             "_lib.load_library(); assert len(_lib.hip_runtimes()) == 1; print('ok')" % root)
+    r = subprocess.run([sys.executable, "-c", first], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stderr[-1500:]
+    later = ("import sys; sys.path.insert(0, %r); from contrafold_gpu_amd import _lib; _lib.load_library()\n"
+             "try:\n    import contrafold_gpu_amd.autograd\nexcept ImportError as e:\n"
+             "    print('refused', 'two HIP runtimes' in str(e))\n" % root)
+    r = subprocess.run([sys.executable, "-c", later], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "refused True" in r.stdout, (r.stdout, r.stderr[-1500:])
+
+
+def test_write_raw_round_trip(tmp_path):
+    """Parameter write-back: load -> write -> load gives the same raw weights
+    and tables bit for bit, for the bundled weights and for perturbed
+    (trained-like) ones; the gradient's raw order matches the file's."""
+    from contrafold_gpu_amd import params
+    names, raw, pmap = params.load_raw()
+    p1 = tmp_path / "a.params"
+    params.write_raw(str(p1), names, raw)
+    n2, r2, m2 = params.load_raw(str(p1))
+    assert n2 == names and r2.tobytes() == raw.tobytes()
+    assert params.flatten(params.load(str(p1))).tobytes() == params.flatten(params.load()).tobytes()
+    rng = np.random.default_rng(5)
+    trained = raw + rng.normal(scale=1e-3, size=raw.size) * np.pi
+    p2 = tmp_path / "b.params"
+    params.write_raw(str(p2), names, trained)
+    n3, r3, m3 = params.load_raw(str(p2))
+    assert r3.tobytes() == trained.tobytes()
+    assert m3.tables(r3).tobytes() == pmap.tables(trained).tobytes()
+    # a step along the raw gradient direction written and read back unchanged
+    g = pmap.raw_grad(rng.normal(size=params.N_TABLE))
+    step = raw - 0.01 * g
+    params.write_raw(str(p2), names, step)
+    assert params.load_raw(str(p2))[1].tobytes() == step.tobytes()
+    with pytest.raises(ValueError):
+        params.write_raw(str(p2), names[:-1], raw)

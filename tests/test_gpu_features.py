@@ -1,7 +1,7 @@
 """GPU feature tests: sampling statistics and constraints, gradients against
 FP64 finite differences, graph replay, launch-shape invariance, serving."""
 import math
-
+import os
 import time
 
 import numpy as np
@@ -10,6 +10,7 @@ import pytest
 from conftest import rand_seq
 
 pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def pairs_of(db):
@@ -157,19 +158,36 @@ def test_gradient_closed_form_unpairable(P):
     assert np.abs(tg).max() < 1e-6
 
 
-def test_torch_autograd(P):
-    torch = pytest.importorskip("torch")
-    import contrafold_gpu_amd as cf
-    from contrafold_gpu_amd import autograd, params
-    rng = np.random.default_rng(9)
-    seqs = [rand_seq(rng, 50) for _ in range(4)]
-    theta = autograd.raw_theta()
-    z = autograd.logz(theta, seqs)
-    z.mean().backward()
-    _z, tg = cf.logZ_grad(seqs, P, weights=np.full(4, 0.25, np.float32))
-    want = params.param_map().raw_grad(tg)
-    assert np.allclose(theta.grad.numpy(), want, rtol=1e-5, atol=1e-7)
-    assert np.allclose(z.detach().numpy(), cf.logZ_batch(seqs, P))
+def test_torch_autograd():
+    """torch.autograd.Function over the outside-pass adjoint.  Runs in a child
+    process that imports torch first, so torch and the engine share one HIP
+    runtime (this process started the engine without torch)."""
+    pytest.importorskip("torch")
+    import subprocess
+    import sys
+    code = r"""
+import sys, numpy as np, torch
+sys.path.insert(0, %r)
+import contrafold_gpu_amd as cf
+from contrafold_gpu_amd import autograd, params, _lib
+from conftest import rand_seq
+assert len(_lib.hip_runtimes()) <= 1
+P = cf.load()
+rng = np.random.default_rng(9)
+seqs = [rand_seq(rng, 50) for _ in range(4)]
+theta = autograd.raw_theta()
+z = autograd.logz(theta, seqs)
+z.mean().backward()
+_z, tg = cf.logZ_grad(seqs, P, weights=np.full(4, 0.25, np.float32))
+want = params.param_map().raw_grad(tg)
+assert np.allclose(theta.grad.numpy(), want, rtol=1e-5, atol=1e-7)
+assert np.allclose(z.detach().numpy(), cf.logZ_batch(seqs, P))
+assert len(_lib.hip_runtimes()) == 1
+print("autograd ok")
+""" % (ROOT,)
+    r = subprocess.run([sys.executable, "-c", code], cwd=os.path.join(ROOT, "tests"), capture_output=True,
+                       text=True, timeout=110)
+    assert r.returncode == 0 and "autograd ok" in r.stdout, (r.stdout[-2000:], r.stderr[-2000:])
 
 
 def test_graph_replay_matches_eager(engine, P):
