@@ -171,27 +171,35 @@ def run_config(name, args):
                                                      or [0.0]))})
     elif name == "latency":
         # single-fold latency floor: one uniform-ACGU sequence alone on the GPU,
-        # one wave (packed layout) vs a 1,024-thread workgroup, device ms per kind
+        # one wave (packed layout), a 1,024-thread workgroup, and split teams
+        # of K 256-thread workgroups (K CUs); device ms per kind
         rng = np.random.default_rng(1)
         rows = []
         for L in (100, 200, 400, 600, 1000):
             seq = "".join(rng.choice(list("ACGU"), L))
-            for w in (64, 1024):
+            for w, K in ((64, 1), (1024, 1), (256, 2), (256, 4), (256, 8)):
                 eng.set_tuning([1 << 30], [w])
+                eng.set_split(1 if K > 1 else 0, K)
                 with eng.batch([seq]) as b:
-                    row = {"L": L, "threads": w}
+                    row = {"L": L, "threads": w, "wgs_per_fold": K}
                     for flags, kw, kind in ((cf.CF_MEA, dict(gammas=[6.0]), "mea"), (cf.CF_BPP, {}, "bpp"),
                                             (cf.CF_SAMPLE, dict(n_samples=10), "sample10")):
                         b.run(flags, recompute=True, **kw)
                         b.run(flags, recompute=True, **kw)
                         row[f"{kind}_ms"] = round(b.last_ms(), 2)
+                        if kind == "mea":
+                            # device ms per stage of the same run, one stage at a time
+                            b.run(flags, recompute=True, timing=True, **kw)
+                            row["mea_stages_ms"] = {k: round(v, 2) for k, v in b.kernel_ms().items() if v > 0.01}
                     rows.append(row)
                     log(f"[latency] {row}")
         eng.set_tuning(*autotune.DEFAULT)
+        eng.set_split(0, 1)
         out["rows"] = rows
     elif name == "c5":
         from contrafold_gpu_amd import serving
-        out.update(serving.benchmark(eng, P, n=args.serve_n, log=log))
+        out.update(serving.benchmark(eng, P, n_plain=6000, seconds=args.serve_seconds, fracs=(0.25, 0.5, 0.8),
+                                     n_streams=args.serve_streams, log=log))
     elif name == "sweep":
         out.update(reference_sweeps(cf, workloads, P, log))
     print(json.dumps(out), flush=True)
@@ -338,6 +346,7 @@ def main():
     ap.add_argument("--check", type=int, default=64, help="sequences the oracle re-folds (gate, c1, c4)")
     ap.add_argument("--serve-n", type=int, default=12000)
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="seconds of arrivals per serving rate")
+    ap.add_argument("--serve-streams", type=int, default=16, help="server slots (HIP streams)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)

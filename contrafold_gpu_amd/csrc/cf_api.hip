@@ -92,6 +92,19 @@ __global__ void cf_copy_kernel(const float4* __restrict__ src, float4* __restric
         dst[i] = src[i];
 }
 
+__global__ void cf_fill_u32(unsigned* p, int64_t n, unsigned v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Zero the barrier counters of a split launch (a fill kernel, not a memset,
+// so the run stays capturable as a hipGraph).
+void clear_xsync(cf_batch* b, const cf_batch::Bucket& bk, hipStream_t st) {
+    const int64_t n = (int64_t)(bk.end - bk.begin) * 16;
+    hipLaunchKernelGGL(cf_fill_u32, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, st,
+                       b->d_xsync + (size_t)bk.begin * 16, n, 0u);
+}
+
 int mea_lds(int Lmax) { return (int)align_up((size_t)(Lmax + 2) * 4 + 2 * (Lmax + 2) + 16, 16) + 2 * (Lmax + 2) * 4 + 16; }
 int vit_lds(int Lmax) { return cf_vit_plan(Lmax).total; }
 
@@ -132,14 +145,20 @@ int launch_inside(cf_batch* b, const Bucket& bk, hipStream_t st) {
     CfLds plan = cf_lds_plan(bk.Lmax, false, 0, G);
     while (plan.total > c->lds_max && G > 1) plan = cf_lds_plan(bk.Lmax, false, 0, G >>= 1);
     if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the inside kernel's LDS plan");
+    const int K = pack ? 1 : bk.split;
+    if (K > 1) clear_xsync(b, bk, st);
     if (pack)
-        hipLaunchKernelGGL((cf_inside_kernel<true>), dim3((nl + G - 1) / G), dim3(64 * G), plan.total, st, b->d_seqs,
+        hipLaunchKernelGGL((cf_inside_kernel<true, false>), dim3((nl + G - 1) / G), dim3(64 * G), plan.total, st, b->d_seqs,
                            b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
-                           b->FCX, bk.Lmax);
+                           b->FCX, bk.Lmax, 1, (unsigned*)nullptr, b->d_status);
+    else if (K > 1)
+        hipLaunchKernelGGL((cf_inside_kernel<false, true>), dim3(cf_split_grid(nl, K)), dim3(bk.threads), plan.total,
+                           st, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
+                           b->FM1, b->FM2, b->FCX, bk.Lmax, K, b->d_xsync + (size_t)bk.begin * 16, b->d_status);
     else
-        hipLaunchKernelGGL((cf_inside_kernel<false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
+        hipLaunchKernelGGL((cf_inside_kernel<false, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
                            b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
-                           b->FCX, bk.Lmax);
+                           b->FCX, bk.Lmax, 1, (unsigned*)nullptr, b->d_status);
     CF_HIP(c, hipGetLastError(), "inside launch");
     return CF_OK;
 }
@@ -171,9 +190,23 @@ int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st) {
         CfLds plan = cf_lds_plan(bk.Lmax, false, 0, G);
         while (plan.total > c->lds_max && G > 1) plan = cf_lds_plan(bk.Lmax, false, 0, G >>= 1);
         if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
-        hipLaunchKernelGGL((cf_outside_kernel<false, true>), dim3((nl + G - 1) / G), dim3(64 * G),
+        hipLaunchKernelGGL((cf_outside_kernel<false, true, false>), dim3((nl + G - 1) / G), dim3(64 * G),
                            plan.total, st, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32,
-                           b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
+                           b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, 1,
+                           (unsigned*)nullptr, b->d_status);
+        CF_HIP(c, hipGetLastError(), "outside launch");
+        return CF_OK;
+    }
+    const int K = bk.split;
+    if (K > 1) {
+        // a split team reads every FCo diagonal from the global table (no ring)
+        CfLds plan = cf_lds_plan(bk.Lmax, false, 0);
+        if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
+        clear_xsync(b, bk, st);
+        hipLaunchKernelGGL((cf_outside_kernel<false, false, true>), dim3(cf_split_grid(nl, K)), dim3(bk.threads),
+                           plan.total, st, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32,
+                           b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, K,
+                           b->d_xsync + (size_t)bk.begin * 16, b->d_status);
         CF_HIP(c, hipGetLastError(), "outside launch");
         return CF_OK;
     }
@@ -182,13 +215,13 @@ int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st) {
     if (!ring) plan = cf_lds_plan(bk.Lmax, false, 0);
     if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
     if (ring)
-        hipLaunchKernelGGL((cf_outside_kernel<true, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
+        hipLaunchKernelGGL((cf_outside_kernel<true, false, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
                            b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
-                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
+                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, 1, (unsigned*)nullptr, b->d_status);
     else
-        hipLaunchKernelGGL((cf_outside_kernel<false, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
+        hipLaunchKernelGGL((cf_outside_kernel<false, false, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
                            b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
-                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax);
+                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, 1, (unsigned*)nullptr, b->d_status);
     CF_HIP(c, hipGetLastError(), "outside launch");
     return CF_OK;
 }
@@ -209,8 +242,11 @@ int launch_post(cf_batch* b, const Bucket& bk, hipStream_t st) {
     while (lds(split) > (size_t)c->lds_max && split > 1) --split;
     if (lds(split) > (size_t)c->lds_max)
         return fail(c, CF_EINVAL, "sequence too long for the posterior kernel's LDS plan");
-    hipLaunchKernelGGL(cf_post_kernel, dim3(nl), dim3(64 * split), lds(split), st, b->d_seqs, b->d_order + bk.begin,
-                       nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->F5, b->F5o, b->FCo, b->FM1o, b->POST, bk.Lmax);
+    // a split team's folds take K workgroups here too (no waits: cells are independent)
+    const int K = bk.threads > 64 ? bk.split : 1;
+    hipLaunchKernelGGL(cf_post_kernel, dim3(K > 1 ? cf_split_grid(nl, K) : nl), dim3(64 * split), lds(split), st,
+                       b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->F5, b->F5o,
+                       b->FCo, b->FM1o, b->POST, bk.Lmax, K);
     CF_HIP(c, hipGetLastError(), "posterior launch");
     return CF_OK;
 }
@@ -541,12 +577,22 @@ size_t batch_staging_bytes(int n, int64_t C) {
         k.take<int64_t>(n);
         k.take<float>(kMaxGamma + n);
         k.take<int>(4);
+        k.take<unsigned>((int64_t)16 * n);
     });
+}
+
+// Workgroups per fold for a wide bucket of nfold folds of up to Lb nt.  The
+// parts of a fold wait for each other, so a split launch is kept to at most
+// two workgroups per CU in total: it fits the device beside other work.
+int bucket_split(const cf_ctx* c, int Lb, int nfold, int split) {
+    int K = split > 0 ? split : ((c->split_min > 0 && Lb >= c->split_min) ? c->split_k : 1);
+    while (K > 1 && (int64_t)cf_split_grid(nfold, K) > 2 * (int64_t)c->n_cu) --K;
+    return K;
 }
 
 int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
                  const uint64_t* keys, hipStream_t stream, void** staging, size_t* staging_bytes, bool sync,
-                 cf_batch** out, int width) {
+                 cf_batch** out, int width, int split) {
     if (!c || !out) return fail(c, CF_EINVAL, "null argument");
     *out = nullptr;
     if (n <= 0) return fail(c, CF_EINVAL, "empty batch");
@@ -594,7 +640,11 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
                 const int64_t work = (int64_t)(end - pos) * Lb * Lb;
                 if (width <= 0 && !c->tuned && w == 64 && work <= (int64_t)83000 * c->n_cu)
                     w = (2 * work <= (int64_t)83000 * c->n_cu) ? 256 : 128;
-                b->buckets.push_back({pos, end, w, Lb});
+                // wide buckets of long folds may split each fold over several
+                // CUs (one 256-thread workgroup, one wave per SIMD, per part)
+                const int K = w > 64 ? bucket_split(c, Lb, end - pos, split) : 1;
+                if (K > 1) w = 256;
+                b->buckets.push_back({pos, end, w, Lb, K});
             }
             pos = end;
         }
@@ -690,6 +740,7 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
     b->d_soff = d.take<int64_t>(n);
     b->d_gin = d.take<float>(kMaxGamma + n);
     b->d_status = d.take<int>(4);
+    b->d_xsync = d.take<unsigned>((int64_t)16 * n);
     for (auto& ev : b->ev) {
         if (hipEventCreate(&ev) != hipSuccess) {
             cf_batch_destroy(b);
@@ -815,9 +866,12 @@ int cf_create(int device, cf_ctx** out) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
         c->lds_max = v;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0) c->n_cu = v;
-    const void* dyn[] = {(const void*)cf_inside_kernel<false>, (const void*)cf_inside_kernel<true>,
-                         (const void*)cf_outside_kernel<true, false>, (const void*)cf_outside_kernel<false, false>,
-                         (const void*)cf_outside_kernel<false, true>, (const void*)cf_post_kernel,
+    const void* dyn[] = {(const void*)cf_inside_kernel<false, false>, (const void*)cf_inside_kernel<true, false>,
+                         (const void*)cf_inside_kernel<false, true>,
+                         (const void*)cf_outside_kernel<true, false, false>,
+                         (const void*)cf_outside_kernel<false, false, false>,
+                         (const void*)cf_outside_kernel<false, true, false>,
+                         (const void*)cf_outside_kernel<false, false, true>, (const void*)cf_post_kernel,
                          (const void*)cf_exterior_kernel,
                          (const void*)cf_mea_kernel, (const void*)cf_viterbi_kernel<false>,
                          (const void*)cf_viterbi_kernel<true>};
@@ -892,6 +946,14 @@ int cf_copy_probe(cf_ctx* c, int64_t bytes) {
     return CF_OK;
 }
 
+int cf_set_split(cf_ctx* c, int min_length, int wgs_per_fold) {
+    if (!c || min_length < 0 || wgs_per_fold < 1 || wgs_per_fold > 16)
+        return fail(c, CF_EINVAL, "split: min_length >= 0 and 1..16 workgroups per fold");
+    c->split_min = wgs_per_fold > 1 ? min_length : 0;
+    c->split_k = wgs_per_fold;
+    return CF_OK;
+}
+
 int cf_set_tuning(cf_ctx* c, const int* edges, const int* threads, int nb) {
     if (!c || !edges || !threads || nb <= 0) return fail(c, CF_EINVAL, "bad tuning arguments");
     std::vector<int> e(edges, edges + nb), t(threads, threads + nb);
@@ -948,9 +1010,14 @@ int cf_batch_run(cf_batch* b, unsigned flags, const float* gammas, int n_gamma, 
         return rc;
     }
     CF_HIP(c, hipStreamSynchronize(b->stream), "fold run");
-    if (ran & CF_SAMPLE) {
+    {
         int st = 0;
         CF_HIP(c, hipMemcpy(&st, b->d_status, sizeof(int), hipMemcpyDeviceToHost), "read status");
+        if (st & 2) {
+            b->done = 0;
+            return fail(c, CF_EDEVICE, "split-team fold: its workgroups were not resident together (results "
+                                       "discarded)");
+        }
         if (st & 1) {
             b->done &= ~(unsigned)CF_SAMPLE;
             return fail(c, CF_EDEVICE, "sampler traceback stack overflow (samples discarded)");
@@ -1080,11 +1147,16 @@ int cf_batch_kernel_ms(const cf_batch* b, float* out) {
 
 int64_t cf_batch_device_bytes(const cf_batch* b) { return b ? b->device_bytes() : 0; }
 
-void cf_batch_destroy(cf_batch* b) {
+void cf_batch_destroy(cf_batch* b) { cfi::batch_free(b, true); }
+
+}  // extern "C"
+
+namespace cfi {
+void batch_free(cf_batch* b, bool sync) {
     if (!b) return;
     cf_ctx* c = b->ctx;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(b->stream);
+    if (sync) (void)hipStreamSynchronize(b->stream);
     for (auto s : b->lanes)
         if (s) (void)hipStreamDestroy(s);
     for (auto e : b->lane_ev)
@@ -1103,5 +1175,4 @@ void cf_batch_destroy(cf_batch* b) {
     region_put(c, b->r_grad);
     delete b;
 }
-
-}  // extern "C"
+}  // namespace cfi

@@ -137,6 +137,14 @@ __device__ __forceinline__ float cf_lse(float a, float b, const float4* __restri
     // NEG (|r| < 1 is far below half an ulp of 1e30), so one test suffices.
     return (d >= 11.8624794162f) ? hi : res;
 }
+// Skipped terms.  The reference skips a term when one of its table parts is
+// not finite (below -5e29).  Every such part is exactly NEG, and a term is
+// that part plus finite scores or plus another NEG, so the term itself is
+// -1e30 or -2e30 (the finite addends vanish in rounding).  cf_lse(acc, t)
+// with t <= -1e30 returns acc bit for bit: for finite acc, d >= 11.86 and hi
+// = acc is returned; for acc = NEG either d >= 11.86 again or d = 0 and NEG +
+// r rounds back to NEG.  So the sweeps fold such terms unconditionally, with
+// no compare or select per term.
 
 // CONTRAfold Fast_Exp for RealT=float (cpu.py:560-580), branch-free.  For
 // x >= 0 the reference calls expf (>= 1); the only consumer here is the

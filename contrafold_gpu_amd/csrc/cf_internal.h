@@ -68,6 +68,9 @@ struct cf_ctx {
     std::vector<int> edges{128, 384, 1 << 30};
     std::vector<int> threads{64, 64, 512};
     bool tuned = false;  // edges / widths set by cf_set_tuning (else the built-in default)
+    // split teams: folds of at least split_min nt in a wide bucket run on
+    // split_k workgroups of 256 threads each (0 = off); see cf_set_split
+    int split_min = 0, split_k = 1;
     int lds_max = 65536;
     int n_cu = 256;
     cfi::DevPool pool;
@@ -89,6 +92,7 @@ struct cf_batch {
     std::vector<int> order;
     struct Bucket {
         int begin, end, threads, Lmax;
+        int split = 1;  // workgroups per fold of the inside / outside / posterior launches
     };
     std::vector<Bucket> buckets;
     int64_t total_C = 0, total_T = 0, total_X = 0, total_chars = 0;
@@ -102,7 +106,9 @@ struct cf_batch {
     int64_t* d_charoff = nullptr;
     int64_t* d_soff = nullptr;
     float* d_gin = nullptr;     // per-run small inputs: gammas (64) then grad weights (n)
-    int* d_status = nullptr;    // per-run kernel status bits (1: sampler stack overflow), zeroed by each run
+    int* d_status = nullptr;    // per-run kernel status bits (1: sampler stack overflow, 2: split team not
+                                // co-resident), zeroed by each run
+    unsigned* d_xsync = nullptr;  // split-team barrier counters: 16 words per sequence (by launch order)
     float *FC = nullptr, *FM = nullptr, *FM1 = nullptr, *FM2 = nullptr, *FCX = nullptr, *F5 = nullptr,
           *F5o = nullptr, *logz = nullptr;
     // outside
@@ -159,9 +165,14 @@ int fail(cf_ctx* c, int code, const std::string& msg);
 // (which must stay untouched until the stream has passed the upload).  With
 // sync = true the upload is waited for and `staging` may be reused at once.
 // width > 0 overrides the tuned team width of every bucket.
+// split > 0 overrides the split-team factor of every wide bucket.
 int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
                  const uint64_t* keys, hipStream_t stream, void** staging, size_t* staging_bytes, bool sync,
-                 cf_batch** out, int width = 0);
+                 cf_batch** out, int width = 0, int split = 0);
+// Release a batch.  sync = false skips waiting for the batch's stream: only
+// for a batch whose work is known to be complete (its done event has
+// passed) on a stream that may already hold later work.
+void batch_free(cf_batch* b, bool sync);
 // Bytes of pinned staging batch_create needs for these sizes.
 size_t batch_staging_bytes(int n, int64_t total_residues);
 // Prepare buffers and per-run inputs, then enqueue the stages of `flags` on

@@ -188,17 +188,40 @@ __device__ void cf_stage_seq(const CfShared& S, const uint8_t* __restrict__ code
 // thread's rank and count within the team.  The wave index is made
 // wave-uniform explicitly so that everything derived from the sequence
 // descriptor (table bases, lengths, loop bounds) lives in scalar registers.
+//
+// Split teams (K > 1, wide layout only): K workgroups fold one sequence
+// together, each on its own CU, so a long fold's anti-diagonals are issued by
+// 4K SIMDs instead of the 4 of one CU.  tid / NT index the cells of the whole
+// team (K * blockDim threads); ltid / lNT index the workgroup, which stages
+// its own copy of the sequence in LDS and runs its own pairable-cell list.
+// Workgroups are dispatched to the 8 XCDs round-robin by launch index, so the
+// K parts of a fold take launch indices that are equal modulo 8 -- one XCD,
+// one L2 -- and a block of 8 K consecutive indices holds 8 folds.
 struct CfTeam {
-    int team, tid, NT, li;
+    int team, tid, NT, li, ltid, lNT, part, K;
 };
 template <bool PACK>
-__device__ __forceinline__ CfTeam cf_team(int G) {
+__device__ __forceinline__ CfTeam cf_team(int G, int K = 1) {
     CfTeam t;
+    t.ltid = threadIdx.x;
+    t.lNT = blockDim.x;
+    t.part = 0;
+    t.K = 1;
     if (PACK) {
         t.team = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
         t.tid = threadIdx.x & (WAVE - 1);
         t.NT = WAVE;
         t.li = blockIdx.x * G + t.team;
+        t.ltid = t.tid;
+        t.lNT = WAVE;
+    } else if (K > 1) {
+        const int g = blockIdx.x, x = g & 7, r = g >> 3;
+        t.team = 0;
+        t.K = K;
+        t.part = r % K;
+        t.li = (r / K) * 8 + x;
+        t.tid = t.part * blockDim.x + threadIdx.x;
+        t.NT = K * blockDim.x;
     } else {
         t.team = 0;
         t.tid = threadIdx.x;
@@ -206,6 +229,54 @@ __device__ __forceinline__ CfTeam cf_team(int G) {
         t.li = blockIdx.x;
     }
     return t;
+}
+// Launch size of a split launch over n folds: whole blocks of 8 folds.
+__host__ __device__ inline int cf_split_grid(int n, int K) { return ((n + 7) / 8) * 8 * K; }
+
+// Barrier of a split team: every workgroup of the fold arrives on the fold's
+// counter and waits until all K have arrived for this generation.  Each wave
+// first waits for its own loads and stores to complete; thread 0 then
+// publishes with an agent-scope release and spins with agent-scope acquires,
+// so the other workgroups' table writes are visible after the barrier.  The
+// wait is bounded (about 2 s of wall clock): if the parts of a fold are not
+// resident together, the part that gives up raises the fold's abort word
+// (ctr[1]) and status bit 2; every part then stops waiting (it still
+// arrives), runs to its end, and the host reports the failure.
+#define CF_XBAR_LIMIT 200000000ull  // wall-clock ticks (100 MHz): 2 s
+struct CfXBar {
+    unsigned* ctr;  // this fold's arrival counter, then its abort word (zeroed before the launch)
+    int* status;
+    int* abort;     // LDS flag shared by the workgroup
+    unsigned gen;
+    int K;
+};
+__device__ __forceinline__ void cf_xbar(CfXBar& x) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads and stores have completed
+    __syncthreads();
+    x.gen += 1;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(x.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!*x.abort) {
+            const unsigned target = x.gen * (unsigned)x.K;
+            const uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(x.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (__hip_atomic_load(x.ctr + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    *x.abort = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                if (wall_clock64() - t0 > CF_XBAR_LIMIT) {
+                    *x.abort = 1;
+                    __hip_atomic_store(x.ctr + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    atomicOr(x.status, 2);
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
 }
 
 __device__ __forceinline__ bool cf_allow(const CfShared& S, int a, int b) {
@@ -227,25 +298,38 @@ __device__ __forceinline__ bool cf_allow(const CfShared& S, int a, int b) {
 //   C: FM1, FM of diagonal s       D: FM1, FM of diagonal s+1
 // Each cell's terms are folded in the reference's order, so every table is
 // bit-identical to a one-diagonal-at-a-time sweep.
-template <bool PACK>
-__global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
+// LAT: the split-team (latency) form: 256-thread workgroups, one wave per
+// SIMD, so a wave has the register room to keep the next batch of loads in
+// flight while it folds the current one (the 1,024-thread form would spill).
+template <bool PACK, bool LAT>
+__global__ void __launch_bounds__(LAT ? 256 : 1024) cf_inside_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
                                                         int nseq, const uint8_t* __restrict__ codes,
                                                         const uint8_t* __restrict__ masks,
                                                         const CfParams<float>* __restrict__ P, float* __restrict__ FC,
                                                         float* __restrict__ FM, float* __restrict__ FM1,
-                                                        float* __restrict__ FM2, float* __restrict__ FCX, int Lmax) {
+                                                        float* __restrict__ FM2, float* __restrict__ FCX, int Lmax,
+                                                        int K, unsigned* __restrict__ xsync,
+                                                        int* __restrict__ status) {
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int xabort;
     const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
-    const CfTeam tm = cf_team<PACK>(G);
+    const CfTeam tm = cf_team<PACK>(G, PACK ? 1 : K);
     const CfLds plan = cf_lds_plan(Lmax, false, 0, G);
     const CfShared S = cf_carve(smem, plan, tm.team);
     cf_stage_params(S, P);
+    if (threadIdx.x == 0) xabort = 0;
     __syncthreads();
     if (tm.li >= nseq) return;
     const CfSeq sd = seqs[order[tm.li]];
     const int L = sd.L;
     const int tid = tm.tid, NT = tm.NT;
-    cf_stage_seq<PACK>(S, codes, masks, sd, tid, NT);
+    cf_stage_seq<PACK>(S, codes, masks, sd, tm.ltid, tm.lNT);
+    // team-wide sync (every part of a split team) and workgroup-local sync
+    CfXBar xb{tm.K > 1 ? xsync + (size_t)tm.li * 16 : nullptr, status, &xabort, 0u, tm.K};
+    auto sync_team = [&]() {
+        if (tm.K > 1) cf_xbar(xb);
+        else cf_team_sync<PACK>();
+    };
 
     const float mb = P->t[CF_O_MB], mu = P->t[CF_O_MU], mp = P->t[CF_O_MP];
     const float* __restrict__ T = P->t;
@@ -279,29 +363,55 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
                 int ib0 = S.dtab[s - 2] + i + 2;
                 int ib1 = S.dtab[s - 1] + i + 2;
                 int t = 2;
-                for (; t + 7 <= s - 2; t += 8) {
-                    float x[8], y0[8], y1[8];
+                int tl = 2;  // split offset of the next load
+                auto load8 = [&](float* x, float* y0, float* y1) {
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         x[u] = cf_ld(fm1_t, ia);
                         y0[u] = cf_ld(fm_t, ib0);
                         y1[u] = cf_ld(fm_t, ib1);
-                        ia += L + 1 - (t + u);
-                        ib0 -= L - (s - (t + u) - 1);
-                        ib1 -= L - (s - (t + u));
+                        ia += L + 1 - (tl + u);
+                        ib0 -= L - (s - (tl + u) - 1);
+                        ib1 -= L - (s - (tl + u));
                     }
+                    tl += 8;
+                };
+                auto fold8 = [&](const float* x, const float* y0, const float* y1) {
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        a0 = cf_lse(a0, (x[u] > CF_HALF && y0[u] > CF_HALF) ? x[u] + y0[u] : CF_NEG, C);
-                        a1 = cf_lse(a1, (x[u] > CF_HALF && y1[u] > CF_HALF) ? x[u] + y1[u] : CF_NEG, C);
+                        a0 = cf_lse(a0, x[u] + y0[u], C);
+                        a1 = cf_lse(a1, x[u] + y1[u], C);
+                    }
+                };
+                const int nfull = (s - 3) >> 3;  // batches of 8 of the t = 2 .. s-2 terms
+                if constexpr (LAT) {
+                    // A long fold's last diagonals have few cells, each a chain of
+                    // hundreds of terms: the next batch's loads are issued before
+                    // the current batch is folded, so one wave hides its own load
+                    // latency (two buffers, ping-pong; fold order unchanged).
+                    float xa[8], ya0[8], ya1[8], xb[8], yb0[8], yb1[8];
+                    if (nfull > 0) load8(xa, ya0, ya1);
+                    for (int k = 0; k < nfull; k += 2) {
+                        if (k + 1 < nfull) load8(xb, yb0, yb1);
+                        fold8(xa, ya0, ya1);
+                        if (k + 1 >= nfull) break;
+                        if (k + 2 < nfull) load8(xa, ya0, ya1);
+                        fold8(xb, yb0, yb1);
+                    }
+                } else {
+                    for (int k = 0; k < nfull; ++k) {
+                        float x[8], y0[8], y1[8];
+                        load8(x, y0, y1);
+                        fold8(x, y0, y1);
                     }
                 }
+                t += 8 * nfull;
                 for (; t <= s - 2; ++t) {
                     const float x = cf_ld(fm1_t, ia);
                     const float y0 = cf_ld(fm_t, ib0);
                     const float y1 = cf_ld(fm_t, ib1);
-                    a0 = cf_lse(a0, (x > CF_HALF && y0 > CF_HALF) ? x + y0 : CF_NEG, C);
-                    a1 = cf_lse(a1, (x > CF_HALF && y1 > CF_HALF) ? x + y1 : CF_NEG, C);
+                    a0 = cf_lse(a0, x + y0, C);
+                    a1 = cf_lse(a1, x + y1, C);
                     ia += L + 1 - t;
                     ib0 -= L - (s - t - 1);
                     ib1 -= L - (s - t);
@@ -309,7 +419,7 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
                 if (inner1) {
                     const float x = cf_ld(fm1_t, ia);
                     const float y1 = cf_ld(fm_t, ib1);
-                    a1 = cf_lse(a1, (x > CF_HALF && y1 > CF_HALF) ? x + y1 : CF_NEG, C);
+                    a1 = cf_lse(a1, x + y1, C);
                 }
             }
 #pragma unroll
@@ -330,10 +440,11 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
             }
         }
         cf_team_sync<PACK>();
-        // ---- B: FC of the pairable cells (hairpin, single-branch window, multiloop)
+        // ---- B: FC of the pairable cells (hairpin, single-branch window, multiloop);
+        // a split team's workgroups each take the cells they listed in A
         const int nal = S.cnt[step & 1];
-        if (tid == 0) S.cnt[(step + 1) & 1] = 0;
-        for (int idx = tid; idx < nal; idx += NT) {
+        if (tm.ltid == 0) S.cnt[(step + 1) & 1] = 0;
+        for (int idx = tm.ltid; idx < nal; idx += tm.lNT) {
             const int e0 = S.alist[idx];
             const int i = e0 & 0x7fff;
             const int d = s + (e0 >> 15);
@@ -367,7 +478,7 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
                                    : (l1 == 1 && l2 == 1) ? sn11 : 0.0f;
                     e = e + sn;
                     if (l1 + l2 == 0) e = e_stack;
-                    acc = cf_lse(acc, (fc > CF_HALF) ? e + fc : CF_NEG, C);
+                    acc = cf_lse(acc, e + fc, C);
                 }
             }
             // Rows p >= i+2: candidates (p ascending, q descending) gathered 8
@@ -393,8 +504,7 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
             // The FC loads of a batch are consumed only in its fold, so all 8 are
             // in flight together.  An empty slot reads cell (0, 0), which the
             // first step set to NEG, so its term is NEG.
-            while (more) {
-                float fv[8], ev[8];
+            auto gather8 = [&](float* fv, float* ev) {
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     while (bits == 0u && p < pend) {
@@ -417,8 +527,33 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
                     ev[u] = e;
                 }
                 more = (bits != 0u) || (p < pend);
+            };
+            auto fold8 = [&](const float* fv, const float* ev) {
 #pragma unroll
-                for (int u = 0; u < 8; ++u) acc = cf_lse(acc, (fv[u] > CF_HALF) ? ev[u] + fv[u] : CF_NEG, C);
+                for (int u = 0; u < 8; ++u) acc = cf_lse(acc, ev[u] + fv[u], C);
+            };
+            if constexpr (LAT) {
+                // wide teams: the next batch is gathered before the current one
+                // is folded (ping-pong buffers), so a lone wave hides its own
+                // load latency; batches are folded in the same order
+                float fa[8], ea[8], fb[8], eb[8];
+                bool ha = more;
+                if (ha) gather8(fa, ea);
+                while (ha) {
+                    const bool hb = more;
+                    if (hb) gather8(fb, eb);
+                    fold8(fa, ea);
+                    if (!hb) break;
+                    ha = more;
+                    if (ha) gather8(fa, ea);
+                    fold8(fb, eb);
+                }
+            } else {
+                while (more) {
+                    float fv[8], ev[8];
+                    gather8(fv, ev);
+                    fold8(fv, ev);
+                }
             }
             const float fm2 = fm2_t[db + i];
             if (fm2 > CF_HALF) {
@@ -433,6 +568,8 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
             fc_t[db + i] = acc;
             fcx_t[cf_xbase(j + 1) + i - 1] = acc;
         }
+        // C and D read FC of diagonals <= s-1 only (finished a step ago); this
+        // sync orders the list counters
         cf_team_sync<PACK>();
         // ---- C, D: FM1 and FM of diagonal s, then of s+1 (each reads its predecessor)
         for (int d = s; d <= s + 1 && d <= L; ++d) {
@@ -467,7 +604,7 @@ __global__ void __launch_bounds__(1024) cf_inside_kernel(const CfSeq* __restrict
                 fm1_t[db + i] = fm1;
                 fm_t[db + i] = fmv;
             }
-            cf_team_sync<PACK>();
+            sync_team();
         }
     }
 }
@@ -648,26 +785,34 @@ __global__ void __launch_bounds__(256) cf_exterior_kernel(const CfSeq* __restric
 //   C: FMo of s-1 completed; FM2o, FM1o of s       D: FM2o, FM1o of s-1
 // Each cell's terms are folded in the reference's order, so every table is
 // bit-identical to a one-diagonal-at-a-time sweep.
-template <bool RING, bool PACK>
-__global__ void __launch_bounds__(1024) cf_outside_kernel(
+template <bool RING, bool PACK, bool LAT>
+__global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ order, int nseq, const uint8_t* __restrict__ codes,
     const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
     const float* __restrict__ FM, const float* __restrict__ FM1, const float* __restrict__ F5,
     const float* __restrict__ F5o, float* __restrict__ FCo, float* __restrict__ FMo, float* __restrict__ FM1o,
-    float* __restrict__ FM2o, int Lmax) {
+    float* __restrict__ FM2o, int Lmax, int K, unsigned* __restrict__ xsync, int* __restrict__ status) {
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int xabort;
     const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
-    const CfTeam tm = cf_team<PACK>(G);
+    // a split team keeps no FCo ring: every part reads every diagonal
+    const CfTeam tm = cf_team<PACK>(G, (PACK || RING) ? 1 : K);
     const CfLds plan = cf_lds_plan(Lmax, RING, 0, G);
     const CfShared S = cf_carve(smem, plan, tm.team);
     cf_stage_params(S, P);
+    if (threadIdx.x == 0) xabort = 0;
     __syncthreads();
     if (tm.li >= nseq) return;
     const CfSeq sd = seqs[order[tm.li]];
     const int L = sd.L;
     const int tid = tm.tid, NT = tm.NT;
     const int lane = tid & (WAVE - 1);
-    cf_stage_seq<PACK>(S, codes, masks, sd, tid, NT);
+    cf_stage_seq<PACK>(S, codes, masks, sd, tm.ltid, tm.lNT);
+    CfXBar xb{tm.K > 1 ? xsync + (size_t)tm.li * 16 : nullptr, status, &xabort, 0u, tm.K};
+    auto sync_team = [&]() {
+        if (tm.K > 1) cf_xbar(xb);
+        else cf_team_sync<PACK>();
+    };
 
     const float mb = P->t[CF_O_MB], mu = P->t[CF_O_MU], mp = P->t[CF_O_MP], ep = P->t[CF_O_EP];
     const float* __restrict__ T = P->t;
@@ -733,23 +878,47 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
             const int vmax = L - 1 - (i0 + d);
             int v = vmax;
             int ox = (int)cf_dbase(L, d + v), oy = (int)cf_dbase(L, v);  // diagonals d + v (FM2o) and v (FM)
-            for (; v - 7 >= 2; v -= 8) {
-                float x[8], y[8];
-                bool g[8];
+            {
+                int vl = v;  // split offset of the next load
+                auto load8 = [&](float* x, float* y, bool* g) {
 #pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    const int vv = v - w;
-                    g[w] = dom & (vv <= L - 1 - b);
-                    x[w] = cf_ld(fm2o_t, ox + (g[w] ? a : 0));
-                    y[w] = cf_ld(fm_t, oy + (g[w] ? b : 0));
-                    ox -= L + 2 - (d + vv);
-                    oy -= L + 2 - vv;
-                }
+                    for (int w = 0; w < 8; ++w) {
+                        const int vv = vl - w;
+                        g[w] = dom & (vv <= L - 1 - b);
+                        x[w] = cf_ld(fm2o_t, ox + (g[w] ? a : 0));
+                        y[w] = cf_ld(fm_t, oy + (g[w] ? b : 0));
+                        ox -= L + 2 - (d + vv);
+                        oy -= L + 2 - vv;
+                    }
+                    vl -= 8;
+                };
+                auto fold8 = [&](const float* x, const float* y, const bool* g) {
 #pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    const bool ok = g[w] & (x[w] > CF_HALF) & (y[w] > CF_HALF);
-                    f1 = cf_lse(f1, ok ? x[w] + y[w] : CF_NEG, C);
+                    for (int w = 0; w < 8; ++w) {
+                        f1 = cf_lse(f1, x[w] + y[w], C);
+                    }
+                };
+                const int nfull = v >= 1 ? (v - 1) / 8 : 0;  // batches while v - 7 >= 2
+                if constexpr (LAT) {
+                    float xa[8], ya[8], xb[8], yb[8];
+                    bool ga[8], gb[8];
+                    if (nfull > 0) load8(xa, ya, ga);
+                    for (int k = 0; k < nfull; k += 2) {
+                        if (k + 1 < nfull) load8(xb, yb, gb);
+                        fold8(xa, ya, ga);
+                        if (k + 1 >= nfull) break;
+                        if (k + 2 < nfull) load8(xa, ya, ga);
+                        fold8(xb, yb, gb);
+                    }
+                } else {
+                    for (int k = 0; k < nfull; ++k) {
+                        float x[8], y[8];
+                        bool g[8];
+                        load8(x, y, g);
+                        fold8(x, y, g);
+                    }
                 }
+                v -= 8 * nfull;
             }
             for (; v >= 2; --v) {
                 const bool g = dom & (v <= L - 1 - b);
@@ -757,8 +926,7 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
                 const float y = cf_ld(fm_t, oy + (g ? b : 0));
                 ox -= L + 2 - (d + v);
                 oy -= L + 2 - v;
-                const bool ok = g & (x > CF_HALF) & (y > CF_HALF);
-                f1 = cf_lse(f1, ok ? x + y : CF_NEG, C);
+                f1 = cf_lse(f1, x + y, C);
             }
             if (dom && fmo > CF_HALF) f1 = cf_lse(f1, fmo, C);
             if (live) {
@@ -791,28 +959,52 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
             const int amax = min(c0 + WAVE - 1, na - 1);
             int u = amax - 1;
             int ox0 = (int)cf_dbase(L, s + u), ox1 = (int)cf_dbase(L, s - 1 + u), oy = (int)cf_dbase(L, u);
-            for (; u - 7 >= 2; u -= 8) {
-                float x0[8], x1[8], y[8];
-                bool g0[8], g1[8];
+            {
+                int ul = u;  // split offset of the next load
+                auto load8 = [&](float* x0, float* x1, float* y, bool* g0, bool* g1) {
 #pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    const int uu = u - w;
-                    g0[w] = dom0 & (uu <= a - 1);
-                    g1[w] = dom1 & (uu <= a - 1);
-                    x0[w] = cf_ld(fm2o_t, ox0 + (g0[w] ? a - uu : 0));
-                    x1[w] = cf_ld(fm2o_t, ox1 + (g1[w] ? a - uu : 0));
-                    y[w] = cf_ld(fm1_t, oy + ((g0[w] | g1[w]) ? a - uu : 0));
-                    ox0 -= L + 2 - (s + uu);
-                    ox1 -= L + 2 - (s - 1 + uu);
-                    oy -= L + 2 - uu;
-                }
+                    for (int w = 0; w < 8; ++w) {
+                        const int uu = ul - w;
+                        g0[w] = dom0 & (uu <= a - 1);
+                        g1[w] = dom1 & (uu <= a - 1);
+                        x0[w] = cf_ld(fm2o_t, ox0 + (g0[w] ? a - uu : 0));
+                        x1[w] = cf_ld(fm2o_t, ox1 + (g1[w] ? a - uu : 0));
+                        y[w] = cf_ld(fm1_t, oy + ((g0[w] | g1[w]) ? a - uu : 0));
+                        ox0 -= L + 2 - (s + uu);
+                        ox1 -= L + 2 - (s - 1 + uu);
+                        oy -= L + 2 - uu;
+                    }
+                    ul -= 8;
+                };
+                auto fold8 = [&](const float* x0, const float* x1, const float* y, const bool* g0, const bool* g1) {
 #pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    const bool ok0 = g0[w] & (x0[w] > CF_HALF) & (y[w] > CF_HALF);
-                    const bool ok1 = g1[w] & (x1[w] > CF_HALF) & (y[w] > CF_HALF);
-                    f0 = cf_lse(f0, ok0 ? x0[w] + y[w] : CF_NEG, C);
-                    f1 = cf_lse(f1, ok1 ? x1[w] + y[w] : CF_NEG, C);
+                    for (int w = 0; w < 8; ++w) {
+                        f0 = cf_lse(f0, x0[w] + y[w], C);
+                        f1 = cf_lse(f1, x1[w] + y[w], C);
+                    }
+                };
+                const int nfull = u >= 1 ? (u - 1) / 8 : 0;  // batches while u - 7 >= 2
+                if constexpr (LAT) {
+                    // wide teams: next batch loaded before the current is folded
+                    float xa0[8], xa1[8], ya[8], xb0[8], xb1[8], yb[8];
+                    bool ga0[8], ga1[8], gb0[8], gb1[8];
+                    if (nfull > 0) load8(xa0, xa1, ya, ga0, ga1);
+                    for (int k = 0; k < nfull; k += 2) {
+                        if (k + 1 < nfull) load8(xb0, xb1, yb, gb0, gb1);
+                        fold8(xa0, xa1, ya, ga0, ga1);
+                        if (k + 1 >= nfull) break;
+                        if (k + 2 < nfull) load8(xa0, xa1, ya, ga0, ga1);
+                        fold8(xb0, xb1, yb, gb0, gb1);
+                    }
+                } else {
+                    for (int k = 0; k < nfull; ++k) {
+                        float x0[8], x1[8], y[8];
+                        bool g0[8], g1[8];
+                        load8(x0, x1, y, g0, g1);
+                        fold8(x0, x1, y, g0, g1);
+                    }
                 }
+                u -= 8 * nfull;
             }
             for (; u >= 2; --u) {
                 const bool g0 = dom0 & (u <= a - 1);
@@ -823,10 +1015,8 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
                 ox0 -= L + 2 - (s + u);
                 ox1 -= L + 2 - (s - 1 + u);
                 oy -= L + 2 - u;
-                const bool ok0 = g0 & (x0 > CF_HALF) & (y > CF_HALF);
-                const bool ok1 = g1 & (x1 > CF_HALF) & (y > CF_HALF);
-                f0 = cf_lse(f0, ok0 ? x0 + y : CF_NEG, C);
-                f1 = cf_lse(f1, ok1 ? x1 + y : CF_NEG, C);
+                f0 = cf_lse(f0, x0 + y, C);
+                f1 = cf_lse(f1, x1 + y, C);
             }
 #pragma unroll
             for (int hi = 0; hi < 2; ++hi) {
@@ -846,9 +1036,10 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
         }
         cf_team_sync<PACK>();
         // ---- B: FCo of the pairable cells of both diagonals; FMo of s completed
+        // (a split team's workgroups each take the cells they listed in A)
         const int nal = S.cnt[step & 1];
-        if (tid == 0) S.cnt[(step + 1) & 1] = 0;
-        for (int idx = tid; idx < nal; idx += NT) {
+        if (tm.ltid == 0) S.cnt[(step + 1) & 1] = 0;
+        for (int idx = tm.ltid; idx < nal; idx += tm.lNT) {
             const int e0 = S.alist[idx];
             const int a = e0 & 0x7fff;
             const int d = s - (e0 >> 15);
@@ -913,8 +1104,7 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
                 if (l1 < l1_last) return;
                 row();
                 bool more = true;
-                while (more) {
-                    float fv[8], ev[8];
+                auto gather8 = [&](float* fv, float* ev) {
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         while (bits == 0u && l1 > l1_last) {
@@ -942,19 +1132,43 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
                         }
                         if constexpr (RING) {
                             const float fco = has ? ring[((d + l1 + l2 + 2) % CF_RING) * RS + ip] : CF_NEG;
-                            fv[u] = (fco > CF_HALF) ? fco + e : CF_NEG;
+                            fv[u] = fco + e;
                         } else {
                             fv[u] = cf_ld(fco_t, off);
                             ev[u] = e;
                         }
                     }
                     more = (bits != 0u) || (l1 > l1_last);
+                };
+                auto fold8 = [&](const float* fv, const float* ev) {
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         if constexpr (RING)
                             acc = cf_lse(acc, fv[u], C);
                         else
-                            acc = cf_lse(acc, (fv[u] > CF_HALF) ? fv[u] + ev[u] : CF_NEG, C);
+                            acc = cf_lse(acc, fv[u] + ev[u], C);
+                    }
+                };
+                if constexpr (LAT) {
+                    // wide teams without the ring: the next batch is gathered
+                    // before the current one is folded (ping-pong buffers)
+                    float fa[8], ea[8], fb[8], eb[8];
+                    gather8(fa, ea);
+                    bool ha = true;
+                    while (ha) {
+                        const bool hb = more;
+                        if (hb) gather8(fb, eb);
+                        fold8(fa, ea);
+                        if (!hb) break;
+                        ha = more;
+                        if (ha) gather8(fa, ea);
+                        fold8(fb, eb);
+                    }
+                } else {
+                    while (more) {
+                        float fv[8], ev[8];
+                        gather8(fv, ev);
+                        fold8(fv, ev);
                     }
                 }
             };
@@ -973,15 +1187,15 @@ __global__ void __launch_bounds__(1024) cf_outside_kernel(
             if (RING) ring[(d % CF_RING) * RS + a] = acc;
         }
         fmo_finish(s);
-        cf_team_sync<PACK>();
+        sync_team();
         // ---- C: FMo of s-1 completed; FM2o, FM1o of s
         if (s >= 1) fmo_finish(s - 1);
         fm12o(s);
-        cf_team_sync<PACK>();
+        sync_team();
         // ---- D: FM2o, FM1o of s-1
         if (s >= 1) {
             fm12o(s - 1);
-            cf_team_sync<PACK>();
+            sync_team();
         }
     }
 }
@@ -1011,15 +1225,23 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
     const CfSeq* __restrict__ seqs, const int* __restrict__ order, int nseq, const uint8_t* __restrict__ codes,
     const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
     const float* __restrict__ F5, const float* __restrict__ F5o, const float* __restrict__ FCo,
-    const float* __restrict__ FM1o, float* __restrict__ POSTF, int Lmax) {
+    const float* __restrict__ FM1o, float* __restrict__ POSTF, int Lmax, int K) {
     extern __shared__ __align__(16) unsigned char smem[];
-    if ((int)blockIdx.x >= nseq) return;
+    // K > 1: K workgroups share a sequence (launch indices as in cf_team) and
+    // split its diagonals; the cells are independent, so they never wait
+    int li = (int)blockIdx.x, part = 0;
+    if (K > 1) {
+        const int g = (int)blockIdx.x, r = g >> 3;
+        part = r % K;
+        li = (r / K) * 8 + (g & 7);
+    }
+    if (li >= nseq) return;
     const int G = (int)(blockDim.x / WAVE);
     const CfLds plan = cf_lds_plan(Lmax, false, 0, 1);
     const CfShared S = cf_carve(smem, plan, 0);
     cf_stage_params(S, P);
     __syncthreads();
-    const CfSeq sd = seqs[order[blockIdx.x]];
+    const CfSeq sd = seqs[order[li]];
     const int L = sd.L;
     cf_stage_seq<false>(S, codes, masks, sd, threadIdx.x, blockDim.x);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
@@ -1194,7 +1416,7 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
 
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int pn = 0;  // pooled cells (wave-uniform)
-    for (int s = wv; s <= L; s += G) {
+    for (int s = part * G + wv; s <= L; s += K * G) {
         const int ncell = L + 1 - s;
         for (int i0 = 0; i0 < ncell; i0 += WAVE) {
             const int a = i0 + lane;
@@ -2426,22 +2648,32 @@ __global__ void __launch_bounds__(256) cf_grad_reduce_kernel(const double* __res
 }
 
 // ------------------------------------------------------------ explicit inits
-template __global__ void cf_inside_kernel<false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
-                                                 const CfParams<float>*, float*, float*, float*, float*, float*, int);
-template __global__ void cf_inside_kernel<true>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
-                                                const CfParams<float>*, float*, float*, float*, float*, float*, int);
-template __global__ void cf_outside_kernel<true, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+template __global__ void cf_inside_kernel<false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                 const CfParams<float>*, float*, float*, float*, float*, float*, int,
+                                                 int, unsigned*, int*);
+template __global__ void cf_inside_kernel<true, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                const CfParams<float>*, float*, float*, float*, float*, float*, int,
+                                                int, unsigned*, int*);
+template __global__ void cf_outside_kernel<true, false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                         const CfParams<float>*, const float*, const float*, const float*,
                                                         const float*, const float*, float*, float*, float*, float*,
-                                                        int);
-template __global__ void cf_outside_kernel<false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                        int, int, unsigned*, int*);
+template __global__ void cf_outside_kernel<false, false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                          const CfParams<float>*, const float*, const float*,
                                                          const float*, const float*, const float*, float*, float*,
-                                                         float*, float*, int);
-template __global__ void cf_outside_kernel<false, true>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                         float*, float*, int, int, unsigned*, int*);
+template __global__ void cf_outside_kernel<false, true, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                         const CfParams<float>*, const float*, const float*, const float*,
                                                         const float*, const float*, float*, float*, float*, float*,
-                                                        int);
+                                                        int, int, unsigned*, int*);
+template __global__ void cf_inside_kernel<false, true>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
+                                                       const CfParams<float>*, float*, float*, float*, float*, float*,
+                                                       int, int, unsigned*, int*);
+template __global__ void cf_outside_kernel<false, false, true>(const CfSeq*, const int*, int, const uint8_t*,
+                                                               const uint8_t*, const CfParams<float>*, const float*,
+                                                               const float*, const float*, const float*, const float*,
+                                                               float*, float*, float*, float*, int, int, unsigned*,
+                                                               int*);
 template __global__ void cf_viterbi_kernel<true>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
                                                  const CfParams<double>*, double*, double*, double*, double*, uint8_t*,
                                                  const int64_t*, int);

@@ -106,9 +106,12 @@ int retire(cf_server* s, Slot& sl) {
     s->run_us += t - sl.launched_us;
     const float* lz = (const float*)sl.out;
     const char* pay = (const char*)sl.out + sizeof(float) * (size_t)b->n;
+    size_t pay_bytes = 0;
+    if (sl.kind == CF_MEA) pay_bytes = (size_t)b->total_chars;
+    else if (sl.kind == CF_SAMPLE) pay_bytes = (size_t)b->n_samples * (size_t)b->total_chars;
+    else if (sl.kind == CF_BPP) pay_bytes = sizeof(float) * (size_t)b->total_T;
     int status = 0;
-    if (sl.kind == CF_SAMPLE)
-        std::memcpy(&status, pay + (size_t)b->n_samples * (size_t)b->total_chars, sizeof(int));
+    std::memcpy(&status, pay + pay_bytes, sizeof(int));
     int64_t toff = 0;
     for (size_t k = 0; k < sl.reqs.size(); ++k) {
         Req& r = s->reqs[sl.reqs[k]];
@@ -267,8 +270,8 @@ int launch(cf_server* s, Slot& sl) {
         if (kind == CF_MEA) src = b->mea_out, pay = (size_t)b->total_chars;
         else if (kind == CF_SAMPLE) src = b->samp_out, pay = (size_t)b->n_samples * b->total_chars;
         else if (kind == CF_BPP) src = b->POST, pay = sizeof(float) * (size_t)b->total_T;
-        // a sampling batch also brings back its kernel status word
-        const size_t stat = kind == CF_SAMPLE ? sizeof(int) : 0;
+        // every batch brings back its kernel status word
+        const size_t stat = sizeof(int);
         const size_t need = sizeof(float) * (size_t)n + pay + stat;
         if (!grow_pinned(s, &sl.out, &sl.out_bytes, need)) rc = cfi::fail(c, CF_ENOMEM, "pinned result buffer");
         if (!rc && hipMemcpyAsync(sl.out, b->logz, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, sl.stream) !=
@@ -278,7 +281,7 @@ int launch(cf_server* s, Slot& sl) {
             hipMemcpyAsync((char*)sl.out + sizeof(float) * (size_t)n, src, pay, hipMemcpyDeviceToHost, sl.stream) !=
                 hipSuccess)
             rc = cfi::fail(c, CF_EDEVICE, "result copy");
-        if (!rc && stat &&
+        if (!rc &&
             hipMemcpyAsync((char*)sl.out + sizeof(float) * (size_t)n + pay, b->d_status, stat, hipMemcpyDeviceToHost,
                            sl.stream) != hipSuccess)
             rc = cfi::fail(c, CF_EDEVICE, "result copy");
@@ -384,7 +387,8 @@ int cf_server_result(cf_server* s, int64_t id, char* structure, float* logz, dou
         s->reqs.erase(it);
         return cfi::fail(s->ctx, code,
                          code == CF_ENOMEM ? "request failed (does not fit the HBM budget or device error)"
-                                           : "request failed (sampler traceback stack overflow)");
+                                           : "request failed (device status: sampler stack overflow or "
+                                             "split-team fold not co-resident)");
     }
     if (r.state != 2) return cfi::fail(s->ctx, CF_ESTATE, "request not complete");
     if (structure && !r.result.empty()) std::memcpy(structure, r.result.data(), r.result.size());

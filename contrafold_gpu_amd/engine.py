@@ -131,6 +131,12 @@ class Engine:
         with self.lock:
             check(self.lib.cf_set_tuning(self.ctx, e, t, len(edges)), self.ctx, "cf_set_tuning")
 
+    def set_split(self, min_length, wgs_per_fold):
+        """Split teams: folds of >= min_length nt in wide buckets run on
+        ``wgs_per_fold`` workgroups (CUs) each (cf_set_split); 1 turns it off."""
+        with self.lock:
+            check(self.lib.cf_set_split(self.ctx, int(min_length), int(wgs_per_fold)), self.ctx, "cf_set_split")
+
     def batch(self, seqs, masks=None, keys=None, sampling=True):
         return Batch(self, seqs, masks, keys, sampling)
 

@@ -227,6 +227,7 @@ def benchmark(engine, P, n=12000, fracs=(0.25, 0.5, 0.7, 0.85, 1.0), seed=5, log
     log(f"[serve] plain batched throughput of the mix: {plain:.0f} folds/s")
     srv = Server(engine, n_streams=n_streams)
     rows = []
+    prev = {}
     try:
         for f in fracs:
             qps = f * plain
@@ -246,12 +247,20 @@ def benchmark(engine, P, n=12000, fracs=(0.25, 0.5, 0.7, 0.85, 1.0), seed=5, log
             # latency by length class: the long tail's own fold time bounds p99
             L = np.asarray(tr["lengths"])
             by = {}
+            run = lat - wait
             for lo, hi in ((0, 100), (100, 200), (200, 400), (400, 700), (700, 1001)):
                 m = ok & (L > lo) & (L <= hi)
                 if m.any():
                     by[f"{lo + 1}-{hi}"] = {"n": int(m.sum()), "p50_ms": round(float(np.percentile(lat[m], 50)), 1),
-                                            "p99_ms": round(float(np.percentile(lat[m], 99)), 1)}
+                                            "p99_ms": round(float(np.percentile(lat[m], 99)), 1),
+                                            "queued_p50_ms": round(float(np.percentile(wait[m], 50)), 1),
+                                            "queued_p99_ms": round(float(np.percentile(wait[m], 99)), 1),
+                                            "run_p50_ms": round(float(np.percentile(run[m], 50)), 1),
+                                            "run_p99_ms": round(float(np.percentile(run[m], 99)), 1)}
             row["by_length"] = by
+            st = srv.stats()
+            row["server"] = {k: (round(v - prev.get(k, 0.0), 2) if k != "max_queue" else v) for k, v in st.items()}
+            prev = st
             rows.append(row)
             log(f"[serve] {row}")
     finally:

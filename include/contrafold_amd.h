@@ -82,6 +82,18 @@ int cf_set_params(cf_ctx *ctx, const double *tables);
  * Defaults are set at cf_create; the autotuner overrides them. */
 int cf_set_tuning(cf_ctx *ctx, const int *edges, const int *threads, int n_buckets);
 
+/* Split teams (no counterpart in the reference, whose kernel gives a fold
+ * one thread block, gpu.py:55-184, 508): in a wide bucket (team width > 64)
+ * folds of at least min_length nt run their inside, outside and posterior
+ * stages on wgs_per_fold workgroups of 256 threads -- one wave per SIMD on
+ * wgs_per_fold CUs -- that meet at a barrier per anti-diagonal, which cuts a
+ * long fold's latency.  Results are bit-identical to the one-workgroup team.
+ * A launch is kept to two workgroups per CU in total (fewer parts per fold
+ * otherwise); if the parts of a fold are still not resident together the
+ * run fails with CF_EDEVICE after a bounded wait instead of hanging.
+ * wgs_per_fold = 1 (or min_length = 0) turns splitting off, the default. */
+int cf_set_split(cf_ctx *ctx, int min_length, int wgs_per_fold);
+
 /* Batch: encode, pack and upload n sequences (gpu.py:482-498). keys: NULL or
  * per-sequence 64-bit ids for the sampler's random streams (default: index). */
 int cf_batch_create(cf_ctx *ctx, int n, const char *seqs, const int64_t *offsets, const uint8_t *masks,
@@ -134,7 +146,10 @@ int cf_copy_probe(cf_ctx *ctx, int64_t bytes);
 /* n_streams (1..32) slots, each with its own stream.  HIP maps streams onto
  * GPU_MAX_HW_QUEUES hardware queues (default 4) and a queue runs in order, so
  * set that environment variable >= n_streams before the process first uses the
- * GPU, or short folds wait behind long ones that share their queue. */
+ * GPU, or short folds wait behind long ones that share their queue.  Measured
+ * on the serving mix: 16 slots; 32 oversubscribe the hardware queues (p50 rose
+ * from ~80 to ~450 ms), and one stream per length class with two batches in
+ * flight serialised each class and fell behind the load. */
 int cf_server_create(cf_ctx *ctx, int64_t hbm_budget, int max_batch, int n_streams, cf_server **out);
 /* kind: CF_MEA, CF_BPP, CF_SAMPLE or CF_LOGZ; returns a request id (>= 0) or
  * a negative status.  seed: the request's 64-bit sampling-stream key (the same

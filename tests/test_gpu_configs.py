@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from conftest import rand_seq
+from test_gpu_parity import domains, same_bits
 from test_oracle_pins import TIE_CASES, tie_params
 
 pytestmark = pytest.mark.gpu
@@ -278,3 +279,77 @@ def test_gradient_bitwise_deterministic(P):
                     ref = g.copy()
                 assert g.tobytes() == ref.tobytes()
         eng.close()
+
+
+# ------------------------------------------------------------ split teams
+SPLIT_LENS = (37, 150, 333, 420, 701)
+
+
+@pytest.fixture(scope="module")
+def split_ref(P):
+    """Unsplit results (one workgroup per fold, 1,024 threads) of the split
+    test set: tables, posteriors, MEA and samples."""
+    from contrafold_gpu_amd import CF_MEA, CF_SAMPLE, Engine
+    rng = np.random.default_rng(61)
+    seqs = [rand_seq(rng, L) for L in SPLIT_LENS]
+    masks = [None, (rng.random(150) < 0.1).astype(np.uint8), None, None, (rng.random(701) < 0.05).astype(np.uint8)]
+    eng = Engine()
+    eng.set_params(P)
+    eng.set_tuning([1 << 30], [1024])
+    with eng.batch(seqs, masks) as b:
+        b.run(CF_MEA | CF_SAMPLE, gammas=[6.0], n_samples=4, seed=2)
+        ref = {"logz": b.logz().copy(), "mea": b.mea(0), "samples": b.samples(),
+               "tables": {(k, t): b.table(k, t) for k in range(len(seqs))
+                          for t in ("FC", "FM", "FM1", "FM2", "FCo", "FMo", "FM1o", "POST")}}
+    eng.close()
+    return seqs, masks, ref
+
+
+@pytest.mark.parametrize("K", [2, 4, 8])
+def test_split_team_bitexact(P, split_ref, K):
+    """Folds run by K workgroups (one per CU) that meet at a barrier per
+    anti-diagonal: every inside / outside table, the posteriors, MEA and the
+    samples equal the one-workgroup team bit for bit."""
+    from contrafold_gpu_amd import CF_MEA, CF_SAMPLE, Engine
+    seqs, masks, ref = split_ref
+    eng = Engine()
+    eng.set_params(P)
+    eng.set_tuning([1 << 30], [1024])
+    eng.set_split(100, K)
+    with eng.batch(seqs, masks) as b:
+        for rep in range(2):
+            b.run(CF_MEA | CF_SAMPLE, gammas=[6.0], n_samples=4, seed=2, recompute=True)
+            assert (bits(b.logz()) == bits(ref["logz"])).all()
+            assert b.mea(0) == ref["mea"] and b.samples() == ref["samples"]
+        for (k, t), want in ref["tables"].items():
+            got = b.table(k, t)
+            same = (bits(got) == bits(want)) | (np.isnan(got) & np.isnan(want))
+            assert same.all(), (SPLIT_LENS[k], t)
+    eng.close()
+
+
+def test_split_team_oracle_420_1100(P, oracle):
+    """Split teams against the oracle at 420 and 1,100 nt (every FP32 table
+    and posterior bit-exact), under hipGraph replay too."""
+    from contrafold_gpu_amd import CF_BPP, Engine
+    from contrafold_gpu_amd._lib import CF_GRAPH
+    rng = np.random.default_rng(63)
+    seqs = [rand_seq(rng, 420), rand_seq(rng, 1100)]
+    eng = Engine()
+    eng.set_params(P)
+    eng.set_tuning([1 << 30], [512])
+    eng.set_split(200, 4)
+    with eng.batch(seqs) as b:
+        b.run(CF_BPP | CF_GRAPH)
+        b.run(CF_BPP | CF_GRAPH)
+        for k, s in enumerate(seqs):
+            L = len(s)
+            z32, POST, T = oracle.posterior32(s, P, tables=True)
+            assert bits(b.logz()[k]) == bits(z32)
+            assert (bits(b.bpp(k)) == bits(POST[1:L + 1, 1:L + 1])).all()
+            fc, fm, _tri = domains(s, None)
+            for name, cells in (("FC", fc), ("FM", fm), ("FM1", fm), ("FCo", fc), ("FMo", fm), ("FM1o", fm)):
+                ii, jj = np.array(cells).T
+                ok = same_bits(b.table(k, name)[ii, jj], T[name][ii, jj])
+                assert ok.all(), (L, name, np.array(cells)[~ok][:5])
+    eng.close()
