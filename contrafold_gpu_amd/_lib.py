@@ -49,6 +49,8 @@ SIGNATURES = [
     ("cf_batch_destroy", None, [_vp]),
     ("cf_fold_bytes", _c.c_int64, [_c.c_int, _c.c_uint, _c.c_int, _c.c_int]),
     ("cf_copy_probe", _c.c_int, [_vp, _c.c_int64]),
+    ("cf_fold_noalloc", _c.c_int, [_vp, _c.c_int, _c.c_char_p, _i64p, _u8p, _c.c_uint, _c.c_float, _f32p,
+                                   _c.c_void_p]),
     ("cf_server_create", _c.c_int, [_vp, _c.c_int64, _c.c_int, _c.c_int, _c.POINTER(_vp)]),
     ("cf_server_submit", _c.c_int64, [_vp, _c.c_char_p, _c.c_int, _u8p, _c.c_uint, _c.c_float, _c.c_int,
                                       _c.c_uint64, _c.c_double]),

@@ -840,6 +840,65 @@ int batch_enqueue(cf_batch* b, unsigned flags, const float* gammas, int n_gamma,
     return CF_OK;
 }
 
+// ------------------------------------------------------------ no-alloc path
+// One fixed device array per GPU for cf_fold_noalloc: the inputs, the DP
+// tables and the results of one call are carved out of it, so the call makes
+// no device allocation.  Its device address is looked up once per context
+// (cf_create, with the context's device current) and used from then on.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Layout of a no-alloc call in the scratch array (Carver over `base`; with
+// base = nullptr it only measures).
+struct NoAllocLayout {
+    CfSeq* seqs;
+    int* order;
+    uint8_t *codes, *masks;
+    int64_t* charoff;
+    float* gin;
+    int* status;
+    unsigned* xsync;
+    float *FC, *FM, *FM1, *FM2, *FCX, *F5, *F5o, *logz, *FCo, *FMo, *FM1o, *FM2o, *POST, *SC;
+    int16_t* TB;
+    uint8_t* mea_out;
+    size_t bytes;
+};
+NoAllocLayout noalloc_layout(char* base, int n, int64_t C, int64_t T, int64_t X, int64_t chars, bool mea) {
+    NoAllocLayout l;
+    Carver k{base};
+    l.seqs = k.take<CfSeq>(n);
+    l.order = k.take<int>(n);
+    l.codes = k.take<uint8_t>(C);
+    l.masks = k.take<uint8_t>(C);
+    l.charoff = k.take<int64_t>(n);
+    l.gin = k.take<float>(kMaxGamma + n);
+    l.status = k.take<int>(4);
+    l.xsync = k.take<unsigned>((int64_t)16 * n);
+    l.FC = k.take<float>(T);
+    l.FM = k.take<float>(T);
+    l.FM1 = k.take<float>(T);
+    l.FM2 = k.take<float>(T);
+    l.FCX = k.take<float>(X);
+    l.F5 = k.take<float>(C);
+    l.F5o = k.take<float>(C);
+    l.logz = k.take<float>(n);
+    l.FCo = l.FMo = l.FM1o = l.FM2o = l.POST = l.SC = nullptr;
+    l.TB = nullptr;
+    l.mea_out = nullptr;
+    if (mea) {
+        l.FCo = k.take<float>(T);
+        l.FMo = k.take<float>(T);
+        l.FM1o = k.take<float>(T);
+        l.FM2o = k.take<float>(T);
+        l.POST = k.take<float>(T);
+        l.SC = k.take<float>(T);
+        l.TB = k.take<int16_t>(T);
+        l.mea_out = k.take<uint8_t>(chars);
+    }
+    l.bytes = align_up(k.off, 256);
+    return l;
+}
+
 }  // namespace cfi
 
 // =================================================================== C ABI
@@ -877,6 +936,11 @@ int cf_create(int device, cf_ctx** out) {
                          (const void*)cf_viterbi_kernel<true>};
     for (const void* f : dyn) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
     (void)hipGetLastError();
+    // the no-alloc path's fixed array: its address on this device (a host-side
+    // use of the symbol's name would not be a device address)
+    void* sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(cfi::scratch)) == hipSuccess) c->scratch = sp;
+    else (void)hipGetLastError();
     *out = c;
     return CF_OK;
 }
@@ -1146,6 +1210,108 @@ int cf_batch_kernel_ms(const cf_batch* b, float* out) {
 }
 
 int64_t cf_batch_device_bytes(const cf_batch* b) { return b ? b->device_bytes() : 0; }
+
+int cf_fold_noalloc(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, const uint8_t* masks,
+                    unsigned flags, float gamma, float* out_logz, char* out_mea) {
+    if (!c) return CF_EINVAL;
+    if (n <= 0 || !offsets || !out_logz || (!seqs && offsets[n] > 0)) return fail(c, CF_EINVAL, "bad arguments");
+    if (flags != CF_LOGZ && flags != CF_MEA) return fail(c, CF_EINVAL, "no-alloc fold: flags must be CF_LOGZ or CF_MEA");
+    const bool mea = flags == CF_MEA;
+    if (mea && (!out_mea || !(gamma > 0.0f) || !std::isfinite(gamma))) return fail(c, CF_EINVAL, "MEA needs gamma > 0");
+    if (!c->dP32) return fail(c, CF_ENOPARAMS, "parameters not set (cf_set_params)");
+    if (!c->scratch) return fail(c, CF_EDEVICE, "no-alloc fold: scratch array address unavailable on this device");
+    CF_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    cf_batch b;  // host-side description only; every device buffer is in the scratch array
+    b.ctx = c;
+    b.stream = c->stream;
+    b.n = n;
+    b.L.resize(n);
+    b.charoff.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const int64_t l = offsets[i + 1] - offsets[i];
+        if (l < 0 || l > 32000) return fail(c, CF_EINVAL, "sequence length out of range");
+        b.L[i] = (int)l;
+        b.charoff[i] = b.total_chars;
+        b.total_chars += l;
+        b.Lmax = std::max(b.Lmax, (int)l);
+    }
+    std::vector<CfSeq> sd(n);
+    int64_t C = 0, T = 0, X = 0;
+    for (int i = 0; i < n; ++i) {
+        sd[i].L = b.L[i];
+        sd[i].coff = (int32_t)C;
+        sd[i].toff = T;
+        sd[i].xoff = X;
+        C += b.L[i] + 2;
+        T += cf_tri_size(b.L[i]);
+        X += std::max<int64_t>(cf_x_size(b.L[i]), 1);
+    }
+    const NoAllocLayout need = noalloc_layout(nullptr, n, C, T, X, b.total_chars, mea);
+    if (need.bytes > kScratchBytes)
+        return fail(c, CF_EINVAL, "no-alloc fold: batch needs " + std::to_string(need.bytes) +
+                                      " bytes of scratch, more than its " + std::to_string(kScratchBytes));
+    const NoAllocLayout l = noalloc_layout((char*)c->scratch, n, C, T, X, b.total_chars, mea);
+    b.total_C = C;
+    b.total_T = T;
+    b.total_X = X;
+    // one bucket, longest first; a wide team per fold for small calls
+    b.order.resize(n);
+    for (int i = 0; i < n; ++i) b.order[i] = i;
+    std::stable_sort(b.order.begin(), b.order.end(), [&](int x, int y) { return b.L[x] > b.L[y]; });
+    const int64_t work = (int64_t)n * b.Lmax * b.Lmax;
+    const int width = work <= (int64_t)83000 * c->n_cu ? 256 : 64;
+    b.buckets.push_back({0, n, width, b.Lmax, 1});
+    // inputs: staged in host memory of this call, copied into the scratch array
+    std::vector<uint8_t> codes((size_t)C), mk((size_t)C, 0);
+    for (int i = 0; i < n; ++i) {
+        uint8_t* cs = codes.data() + sd[i].coff;
+        uint8_t* ms = mk.data() + sd[i].coff;
+        const int L = b.L[i];
+        cs[0] = cs[L + 1] = 4;
+        for (int q = 0; q < L; ++q) {
+            cs[q + 1] = encode_base(seqs[offsets[i] + q]);
+            ms[q + 1] = (masks && masks[offsets[i] + q]) ? 1 : 0;
+        }
+    }
+    std::vector<float> gin(kMaxGamma + n, 1.0f);
+    gin[0] = mea ? gamma : 1.0f;
+    const int zero[4] = {0, 0, 0, 0};
+    hipStream_t st = c->stream;
+    CF_HIP(c, hipMemcpyAsync(l.seqs, sd.data(), sizeof(CfSeq) * n, hipMemcpyHostToDevice, st), "upload");
+    CF_HIP(c, hipMemcpyAsync(l.order, b.order.data(), sizeof(int) * n, hipMemcpyHostToDevice, st), "upload");
+    CF_HIP(c, hipMemcpyAsync(l.codes, codes.data(), (size_t)C, hipMemcpyHostToDevice, st), "upload");
+    if (masks) CF_HIP(c, hipMemcpyAsync(l.masks, mk.data(), (size_t)C, hipMemcpyHostToDevice, st), "upload");
+    CF_HIP(c, hipMemcpyAsync(l.charoff, b.charoff.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st), "upload");
+    CF_HIP(c, hipMemcpyAsync(l.gin, gin.data(), sizeof(float) * gin.size(), hipMemcpyHostToDevice, st), "upload");
+    CF_HIP(c, hipMemcpyAsync(l.status, zero, sizeof(zero), hipMemcpyHostToDevice, st), "upload");
+    b.d_seqs = l.seqs;
+    b.d_order = l.order;
+    b.d_codes = l.codes;
+    b.d_masks = masks ? l.masks : nullptr;
+    b.d_charoff = l.charoff;
+    b.d_gin = l.gin;
+    b.d_status = l.status;
+    b.d_xsync = l.xsync;
+    b.FC = l.FC; b.FM = l.FM; b.FM1 = l.FM1; b.FM2 = l.FM2; b.FCX = l.FCX;
+    b.F5 = l.F5; b.F5o = l.F5o; b.logz = l.logz;
+    b.FCo = l.FCo; b.FMo = l.FMo; b.FM1o = l.FM1o; b.FM2o = l.FM2o; b.POST = l.POST;
+    b.SC = l.SC; b.TB = l.TB; b.mea_out = l.mea_out;
+    b.n_gamma = mea ? 1 : 0;
+    const unsigned need_flags = mea ? (CF_LOGZ | CF_BPP | CF_MEA) : CF_LOGZ;
+    int rc = enqueue_bucket(&b, b.buckets[0], need_flags, st);
+    if (rc) {
+        (void)hipStreamSynchronize(st);
+        return rc;
+    }
+    CF_HIP(c, hipMemcpyAsync(out_logz, l.logz, sizeof(float) * n, hipMemcpyDeviceToHost, st), "read logz");
+    if (mea && b.total_chars)
+        CF_HIP(c, hipMemcpyAsync(out_mea, l.mea_out, (size_t)b.total_chars, hipMemcpyDeviceToHost, st), "read mea");
+    int status = 0;
+    CF_HIP(c, hipMemcpyAsync(&status, l.status, sizeof(int), hipMemcpyDeviceToHost, st), "read status");
+    CF_HIP(c, hipStreamSynchronize(st), "no-alloc fold");
+    if (status) return fail(c, CF_EDEVICE, "no-alloc fold: device status " + std::to_string(status));
+    return CF_OK;
+}
 
 void cf_batch_destroy(cf_batch* b) { cfi::batch_free(b, true); }
 

@@ -76,6 +76,7 @@ struct cf_ctx {
     cfi::DevPool pool;
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    void* scratch = nullptr;  // device address of the no-alloc path's fixed array on this device
 };
 
 struct Region {

@@ -131,6 +131,20 @@ void cf_batch_destroy(cf_batch *b);
 /* Device bytes a fold of length L needs for the given flags (admission). */
 int64_t cf_fold_bytes(int L, unsigned flags, int n_gamma, int n_samples);
 
+/* Fold without allocating: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same work and results as
+ * cf_batch_create + cf_batch_run(flags) + cf_batch_logz / cf_batch_mea (index
+ * 0) + cf_batch_destroy, for flags CF_LOGZ or CF_MEA (one gamma), but every
+ * device buffer -- inputs, DP tables, results -- is carved out of one fixed
+ * 8 MiB device array that the library holds on each GPU until the process
+ * ends; a call whose tables do not fit (about 8 folds of 200 nt, or one of
+ * 590 nt, with CF_MEA) is rejected with CF_EINVAL.  The call makes no
+ * hipMalloc / hipFree and is synchronous on the context stream.  All calls
+ * on one GPU share the array: calls that overlap on one GPU race. */
+int cf_fold_noalloc(cf_ctx *ctx, int n, const char *seqs, const int64_t *offsets, const uint8_t *masks,
+                    unsigned flags, float gamma, float *out_logz, char *out_mea);
+
 /* Diagnostic, no counterpart in the reference: one device-to-device copy
  * kernel over `bytes` (a positive multiple of 16; read once, written once) on
  * the context stream, synchronous.  It is the known-size reference against
