@@ -2105,7 +2105,13 @@ struct CfRng {
             ++ctr;
             have = 4;
         }
-        const uint32_t x = (have == 4) ? buf.x : (have == 3) ? buf.y : (have == 2) ? buf.z : buf.w;
+        // take the words in the order x, y, z, w by shifting the buffer down:
+        // a lane-varying pick among the four would be compiled as a dynamic
+        // index into a private-memory copy (scratch writes per refill)
+        const uint32_t x = buf.x;
+        buf.x = buf.y;
+        buf.y = buf.z;
+        buf.z = buf.w;
         --have;
         return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f);  // (0, 1)
     }
@@ -2165,9 +2171,11 @@ __device__ __forceinline__ int cf_scan_pick(float p, bool valid, float r, float&
 #define CF_SAMP_STK 128
 #define CF_SAMP_WAVES 8
 __host__ __device__ inline int cf_samp_row(int lmax) { return (lmax + 2 + 3) & ~3; }
+// Per-wave LDS slice: the stack top, and in the local form the F5 row (4n),
+// the codes (n), the structure being drawn (n) and the mask (n, if any).
 __host__ __device__ inline int cf_samp_pitch(int lmax, bool local, bool masks) {
     const int n = cf_samp_row(lmax);
-    return CF_SAMP_STK * 4 + (local ? 5 * n + (masks ? n : 0) : 0);
+    return CF_SAMP_STK * 4 + (local ? 6 * n + (masks ? n : 0) : 0);
 }
 
 // Eight waves per SIMD (the register cap spills a little cold state) beat
@@ -2207,7 +2215,8 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
     uint32_t* lstk = (uint32_t*)slice;
     float* lf5 = (float*)(slice + CF_SAMP_STK * 4);
     uint8_t* lcod = slice + CF_SAMP_STK * 4 + 4 * cf_samp_row(lmax);
-    uint8_t* lmsk = lcod + cf_samp_row(lmax);
+    uint8_t* lstr = lcod + cf_samp_row(lmax);
+    uint8_t* lmsk = lstr + cf_samp_row(lmax);
     const float* __restrict__ T = sp.t;
     const float* __restrict__ JBt = sp.JB;
     const float mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
@@ -2247,8 +2256,20 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
         const float* __restrict__ fm_t = FM + sd.toff;
         const float* __restrict__ fm1_t = FM1 + sd.toff;
         const float* __restrict__ fm2_t = FM2 + sd.toff;
+        // The structure is drawn into the wave's LDS slice (local form) and
+        // written out once, 64 consecutive bytes per store; without the slice
+        // one lane records each pair in global memory.  Either way a pair
+        // costs one byte store per bracket, not one per lane.
         uint8_t* __restrict__ o = out + out_off[b] + (int64_t)smp * L;
-        for (int q = lane; q < L; q += WAVE) o[q] = '.';
+        uint8_t* __restrict__ ostr = kLocal ? lstr : o;
+        for (int q = lane; q < L; q += WAVE) ostr[q] = '.';
+        if constexpr (kLocal) cf_team_sync<true>();
+        auto put_pair = [&](int a, int c) {  // brackets at 0-based a and c
+            if (lane == 0) {
+                ostr[a] = '(';
+                ostr[c] = ')';
+            }
+        };
         CfRng rng;
         const uint64_t key = keys ? keys[b] : (uint64_t)b;
         rng.key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32) ^ (uint32_t)(key >> 32));
@@ -2348,8 +2369,7 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
                 if (pick < 0) {
                     push(0, 0, j - 1);
                 } else {
-                    o[pick] = '(';
-                    o[j - 1] = ')';
+                    put_pair(pick, j - 1);
                     push(1, pick + 1, j - 1);
                     push(0, 0, pick);
                 }
@@ -2419,8 +2439,7 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
                 }
                 if (pick >= 0) {
                     const int pp = i + wl1[pick], q = j - wl2[pick];
-                    o[pp] = '(';
-                    o[q - 1] = ')';
+                    put_pair(pp, q - 1);
                     push(1, pp + 1, q - 1);
                 }
             } else if (typ == 2) {  // FM(i, j): split, j unpaired, or one branch (in that order)
@@ -2459,13 +2478,17 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
                     paired = (r <= c) || !(wun > CF_HALF);
                 }
                 if (paired) {
-                    o[i] = '(';
-                    o[j - 1] = ')';
+                    put_pair(i, j - 1);
                     push(1, i + 1, j - 1);
                 } else if (wun > CF_HALF) {
                     push(3, i + 1, j);
                 }
             }
+        }
+        if constexpr (kLocal) {
+            cf_team_sync<true>();  // lane 0's brackets precede the copy
+            for (int q = lane; q < L; q += WAVE) o[q] = lstr[q];
+            cf_team_sync<true>();  // the copy's reads precede the next structure
         }
         if (overflow && lane == 0) atomicOr(status, 1);
     }

@@ -353,3 +353,27 @@ def test_split_team_oracle_420_1100(P, oracle):
                 ok = same_bits(b.table(k, name)[ii, jj], T[name][ii, jj])
                 assert ok.all(), (L, name, np.array(cells)[~ok][:5])
     eng.close()
+
+
+def test_sampler_local_and_global_forms_agree(P):
+    """The sampler draws into the wave's LDS slice when the batch's longest
+    fold lets four workgroups share a CU, else straight into global memory;
+    a sequence's samples are the same in both forms (and valid)."""
+    import contrafold_gpu_amd as cf
+    rng = np.random.default_rng(73)
+    short = [rand_seq(rng, 120), rand_seq(rng, 200), rand_seq(rng, 37)]
+    masks = [None, (rng.random(200) < 0.1).astype(np.uint8), None]
+    a = cf.sample_batch(short, P, 20, forced_list=masks, seed=9)
+    b = cf.sample_batch(short + [rand_seq(rng, 1800)], P, 20, forced_list=masks + [None], seed=9)
+    assert b[:3] == a
+    for s, m, smp in zip(short, masks, a):
+        for db in smp:
+            st = []
+            for t, ch in enumerate(db):
+                if ch == "(":
+                    st.append(t)
+                elif ch == ")":
+                    x = st.pop()
+                    assert (s[x], s[t]) in {("A", "U"), ("U", "A"), ("C", "G"), ("G", "C"), ("G", "U"), ("U", "G")}
+                    assert m is None or (m[x] == 0 and m[t] == 0)
+            assert not st
