@@ -169,7 +169,7 @@ int launch_exterior(cf_batch* b, const Bucket& bk, int mode, hipStream_t st) {
     if (nl <= 0) return CF_OK;
     // one wave per (sequence, chain); up to four waves per workgroup as LDS allows
     const int per = cf_ext_wave_bytes(bk.Lmax);
-    const int avail = c->lds_max - 2048;  // static tables
+    const int avail = c->lds_max - 4096;  // static tables (LSE coefficients and bins, junction terms: 2.1 KB)
     if (per > avail) return fail(c, CF_EINVAL, "sequence too long for the exterior kernel");
     const int nw = std::max(1, std::min(4, avail / per));
     const int tasks = nl * (mode == 3 ? 2 : 1);

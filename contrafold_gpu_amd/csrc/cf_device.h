@@ -109,20 +109,47 @@ __constant__ const float cf_lse_coef[8][4] = {
     {-0.0000113994f, 0.0003734731f, 0.9959107193f, 0.0149855051f},
 };
 
+// The 7 interior breakpoints of those segments: segment k holds
+// bp[k-1] <= d < bp[k] (the reference's `if (x < bp)` tests).
+__constant__ const float cf_lse_bp[7] = {0.6615367791f, 1.6320158198f, 2.4912588184f, 3.3792499610f,
+                                         4.4261691294f, 5.7890710412f, 7.8162726752f};
+// Segment lookup by quarter-unit bins of d: bin b covers [b/4, (b+1)/4) and
+// holds at most one breakpoint (they are more than 1/4 apart).  Entry b =
+// {breakpoint in the bin as float bits (or +inf), byte offset of the segment
+// below it}; the segment from the breakpoint on is the next one.  Bin 32
+// stands for every d >= 8 (the last segment).  Staged in LDS right after the
+// 8 coefficient rows (cf_lse_stage); 8-byte entries keep the sweeps' LDS plan
+// within six workgroups per CU at 200 nt.
+#define CF_LSE_BINS 33
+__device__ inline void cf_lse_stage(float4* C, int t) {
+    if (t < 8) C[t] = make_float4(cf_lse_coef[t][0], cf_lse_coef[t][1], cf_lse_coef[t][2], cf_lse_coef[t][3]);
+    if (t < CF_LSE_BINS) {
+        const float lo = 0.25f * (float)t, hi = 0.25f * (float)(t + 1);
+        int below = 0;
+        float in_bin = __int_as_float(0x7f800000);  // +inf: no breakpoint in this bin
+        for (int k = 0; k < 7; ++k) {
+            if (cf_lse_bp[k] < lo) ++below;
+            else if (cf_lse_bp[k] < hi) in_bin = cf_lse_bp[k];
+        }
+        if (t == CF_LSE_BINS - 1) in_bin = __int_as_float(0x7f800000);
+        ((int2*)(C + 8))[t] = make_int2(__float_as_int(in_bin), 16 * below);
+    }
+}
+// LDS bytes of the staged table (coefficients + bins)
+#define CF_LSE_TABLE_BYTES (8 * 16 + ((CF_LSE_BINS * 8 + 15) / 16) * 16)
+
 // log(exp(a)+exp(b)), float32, branch-free.  Same result bits as the
-// reference's branchy lse32 (cpu.py:527-557): the interval of d is found by a
-// three-level compare tree over the 7 breakpoints, the cubic's coefficients
-// come from a table in LDS, and the two early exits become one select.
+// reference's branchy lse32 (cpu.py:527-557): the interval of d is found from
+// its quarter-unit bin (d * 4 is exact) and one compare with the bin's
+// breakpoint, the cubic's coefficients come from a table in LDS, and the two
+// early exits become one select.
 __device__ __forceinline__ float cf_lse(float a, float b, const float4* __restrict__ C) {
     const float hi = fmaxf(a, b);
     const float lo = fminf(a, b);
     const float d = hi - lo;
-    const bool b2 = d >= 3.3792499610f;
-    const bool b1 = d >= (b2 ? 5.7890710412f : 1.6320158198f);
-    const float t0 = b2 ? (b1 ? 7.8162726752f : 4.4261691294f) : (b1 ? 2.4912588184f : 0.6615367791f);
-    const bool b0 = d >= t0;
-    // byte offset of the interval's coefficients (three selects and one add)
-    const int off = (b2 ? 64 : 0) + (b1 ? 32 : 0) + (b0 ? 16 : 0);
+    const int bin = (int)fminf(d * 4.0f, (float)(CF_LSE_BINS - 1));
+    const int2 e = ((const int2*)(C + 8))[bin];
+    const int off = e.y + ((d >= __int_as_float(e.x)) ? 16 : 0);
     const float4 c = *(const float4*)((const char*)C + off);
     float r = c.x * d;
     r = r + c.y;

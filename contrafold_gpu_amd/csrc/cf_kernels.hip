@@ -42,7 +42,7 @@ __host__ __device__ inline CfLds cf_lds_plan(int Lmax, bool ring, int dbuf_elem,
     CfLds p;
     p.W = ((Lmax + 2 + 31) >> 5) + 1;
     int o = 0;
-    p.o_lse = o; o += 8 * 16;
+    p.o_lse = o; o += CF_LSE_TABLE_BYTES;  // coefficients, then the segment bins
     p.o_exp = o; o += 8 * 16;
     p.o_cs = o; o += 961 * 4;
     p.o_jb = o; o += 625 * 4;
@@ -139,7 +139,7 @@ __device__ __forceinline__ void cf_team_sync() {
 // part; the caller synchronises the workgroup afterwards).
 __device__ void cf_stage_params(const CfShared& S, const CfParams<float>* __restrict__ P) {
     const int tid = threadIdx.x, NT = blockDim.x;
-    for (int t = tid; t < 8; t += NT) S.lse[t] = make_float4(cf_lse_coef[t][0], cf_lse_coef[t][1], cf_lse_coef[t][2], cf_lse_coef[t][3]);
+    for (int t = tid; t < CF_LSE_BINS; t += NT) cf_lse_stage(S.lse, t);
     for (int t = tid; t < 6; t += NT) S.ex[t] = make_float4(cf_exp_coef[t][0], cf_exp_coef[t][1], cf_exp_coef[t][2], cf_exp_coef[t][3]);
     for (int t = tid; t < 961; t += NT) S.cs[t] = P->t[CF_O_CS + t];
     for (int t = tid; t < 625; t += NT) S.JB[t] = P->JB[t];
@@ -654,11 +654,11 @@ __global__ void __launch_bounds__(256) cf_exterior_kernel(const CfSeq* __restric
                                                           const float* __restrict__ FCX, float* __restrict__ F5,
                                                           float* __restrict__ F5o, float* __restrict__ logz,
                                                           int mode, int Lmax) {
-    __shared__ float4 C[8];
+    __shared__ float4 C[CF_LSE_TABLE_BYTES / 16];  // cf_lse's coefficients and segment bins
     __shared__ float tabs[25 + 25 + 125 + 125];
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
-    if (tid < 8) C[tid] = make_float4(cf_lse_coef[tid][0], cf_lse_coef[tid][1], cf_lse_coef[tid][2], cf_lse_coef[tid][3]);
+    for (int t = tid; t < CF_LSE_BINS; t += blockDim.x) cf_lse_stage(C, t);
     for (int t = tid; t < 300; t += blockDim.x) {
         float v;
         if (t < 25) v = P->t[CF_O_BP + t];
