@@ -113,9 +113,16 @@ int vit_lds(int Lmax) { return cf_vit_plan(Lmax).total; }
 // sequences per workgroup, one wave each, sharing the staged parameter
 // tables.  Wider buckets give the whole workgroup to one sequence.
 constexpr int kPack = 4;
-// sequences per workgroup of the packed inside / outside sweeps (8 per
-// workgroup measured no faster at 200 nt); fewer when LDS is short
-constexpr int kPackIn = 4;
+// sequences per workgroup of the packed inside / outside sweeps; fewer when
+// LDS is short.  Alone, at 10,000 x 200 nt (inside / outside ms): 1 per
+// workgroup 90.7 / 120.3, 2: 65.4 / 87.1, 4: 58.3 / 78.8, 8: 57.0 / 77.0 --
+// eight one-wave teams per workgroup keep 32 teams on a CU (LDS-limited)
+// against 24, and more resident folds hide more of the split reads' latency.
+// But the outside sweep shares the GPU with the sampler in a logZ + bpp +
+// samples run, and there its 512-thread, 40 KB workgroups found CU room less
+// often: 80.7 -> 104.7 ms and 58.2k -> 53.3k folds/s.  So the inside packs
+// eight and the outside four.
+constexpr int kPackIn = 8;
 constexpr int kPackOut = 4;
 constexpr int kLanes = 4;  // streams a multi-bucket batch forks onto
 
