@@ -130,8 +130,10 @@ def run_config(name, args):
                 ms.append(b.last_ms())
             dt = (time.perf_counter() - t) / args.steps
         out.update({"folds_per_s": round(len(seqs) / dt, 1), "device_ms": round(float(np.mean(ms)), 2),
-                    "tuning": {"edges": tuned["edges"], "threads": tuned["threads"]}, "graph": True})
+                    "tuning": {"edges": tuned["edges"], "threads": tuned["threads"], "packing": tuned["packing"]},
+                    "graph": True})
         eng.set_tuning(*autotune.DEFAULT)
+        eng.set_packing(*autotune.PACKINGS[0])
     elif name == "c4":
         # 512 x 150 nt: gradient of mean logZ w.r.t. the 708 raw weights vs FP64 central differences
         from contrafold_gpu_amd import params

@@ -30,6 +30,7 @@ SIGNATURES = [
     ("cf_set_params", _c.c_int, [_vp, _f64p]),
     ("cf_set_tuning", _c.c_int, [_vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.c_int]),
     ("cf_set_split", _c.c_int, [_vp, _c.c_int, _c.c_int]),
+    ("cf_set_packing", _c.c_int, [_vp, _c.c_int, _c.c_int]),
     ("cf_batch_create", _c.c_int, [_vp, _c.c_int, _c.c_char_p, _i64p, _u8p, _c.POINTER(_c.c_uint64),
                                    _c.POINTER(_vp)]),
     ("cf_batch_run", _c.c_int, [_vp, _c.c_uint, _f32p, _c.c_int, _c.c_int, _c.c_uint64, _f32p]),

@@ -1,8 +1,11 @@
-"""Autotuner for length buckets and workgroup sizes.
+"""Autotuner for length buckets, workgroup sizes and packing.
 
 A tuning is a list of length edges and one workgroup size per bucket
-(cf_set_tuning).  ``tune`` times each candidate on a sample of the workload
-with the requested stages and installs the fastest; results are cached per
+(cf_set_tuning), and, when a bucket is packed (width 64), the one-wave teams
+per workgroup of the packed inside and outside sweeps (cf_set_packing), which
+sets each workgroup's LDS plan and how many folds a CU holds.  ``tune`` times
+each candidate on a sample of the workload with the requested stages and
+installs the fastest; results are cached per
 (device, workload length profile, stages) in a JSON file ($CF_TUNE_CACHE or
 ~/.cache/contrafold_gpu_amd/tuning.json).
 """
@@ -32,6 +35,8 @@ CANDIDATES = [
     ([BIG], [256]),
 ]
 DEFAULT = CANDIDATES[0]  # the library's built-in tuning
+# (inside, outside) one-wave teams per workgroup; the first is the default
+PACKINGS = [(8, 4), (4, 4), (8, 8), (16, 4), (4, 2), (2, 2)]
 
 
 def cache_path():
@@ -67,8 +72,9 @@ def _save_cache(data):
 
 
 def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, sample=1500, repeats=2,
-         candidates=None, use_cache=True, seed=0, log=None):
-    """Pick (edges, threads) for this workload; installs and returns it.
+         candidates=None, packings=None, use_cache=True, seed=0, log=None):
+    """Pick (edges, threads), then the packing, for this workload; installs
+    and returns them.
 
     The sample is at least a third of the workload (up to 8,000 sequences):
     a small sample leaves the GPU half empty, where wide teams win on latency,
@@ -85,12 +91,13 @@ def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, samp
     cache = _load_cache() if use_cache else {}
     if key in cache:
         edges, threads = cache[key]["edges"], cache[key]["threads"]
+        packing = tuple(cache[key].get("packing", PACKINGS[0]))
         engine.set_tuning(edges, threads)
-        return {"edges": edges, "threads": threads, "ms": cache[key].get("ms"), "cached": True}
-    best = None
-    results = []
-    for edges, threads in (candidates or CANDIDATES):
-        engine.set_tuning(edges, threads)
+        engine.set_packing(*packing)
+        return {"edges": edges, "threads": threads, "packing": list(packing), "ms": cache[key].get("ms"),
+                "cached": True}
+
+    def timed():
         with engine.batch(sub, subm) as b:
             kw = dict(recompute=True)
             if flags & CF_MEA:
@@ -102,16 +109,35 @@ def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, samp
             for _ in range(repeats):
                 b.run(flags, **kw)
                 t.append(b.last_ms())
-        ms = float(np.median(t))
-        results.append((ms, edges, threads))
+        return float(np.median(t))
+
+    packings = list(packings or PACKINGS)
+    engine.set_packing(*packings[0])
+    best = None
+    results = []
+    for edges, threads in (candidates or CANDIDATES):
+        engine.set_tuning(edges, threads)
+        ms = timed()
+        results.append((ms, edges, threads, packings[0]))
         if log:
             log(f"[tune] edges={edges} threads={threads}: {ms:.2f} ms")
         if best is None or ms < best[0]:
-            best = (ms, edges, threads)
-    ms, edges, threads = best
+            best = (ms, edges, threads, packings[0])
+    edges, threads = best[1], best[2]
     engine.set_tuning(edges, threads)
+    if 64 in threads:  # a packed bucket: try the other packings on the chosen buckets
+        for pk in packings[1:]:
+            engine.set_packing(*pk)
+            ms = timed()
+            results.append((ms, edges, threads, pk))
+            if log:
+                log(f"[tune] packing={pk}: {ms:.2f} ms")
+            if ms < best[0]:
+                best = (ms, edges, threads, pk)
+    ms, edges, threads, packing = best
+    engine.set_packing(*packing)
     if use_cache:
-        cache[key] = {"edges": edges, "threads": threads, "ms": ms, "time": time.time()}
+        cache[key] = {"edges": edges, "threads": threads, "packing": list(packing), "ms": ms, "time": time.time()}
         _save_cache(cache)
-    return {"edges": edges, "threads": threads, "ms": ms, "cached": False,
-            "tried": [{"ms": m, "edges": e, "threads": t} for m, e, t in results]}
+    return {"edges": edges, "threads": threads, "packing": list(packing), "ms": ms, "cached": False,
+            "tried": [{"ms": m, "edges": e, "threads": t, "packing": list(p)} for m, e, t, p in results]}

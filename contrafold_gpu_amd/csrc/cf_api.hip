@@ -105,25 +105,40 @@ void clear_xsync(cf_batch* b, const cf_batch::Bucket& bk, hipStream_t st) {
                        b->d_xsync + (size_t)bk.begin * 16, n, 0u);
 }
 
+// Persistent packed launch: workgroups the device holds at once (at most
+// one per G folds), and the fold queue's counter cleared in stream order.
+template <typename Kern>
+int persistent_grid(cf_ctx* c, Kern kern, int G, size_t lds, int nfold) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * G, lds) != hipSuccess || per_cu <= 0) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    return (int)std::max<int64_t>(1, std::min<int64_t>((nfold + G - 1) / G, (int64_t)per_cu * c->n_cu));
+}
+int* clear_queue(cf_batch* b, const cf_batch::Bucket& bk, int stage, hipStream_t st) {
+    int* q = b->d_queue + (size_t)bk.begin * 2 + stage;
+    hipLaunchKernelGGL(cf_fill_u32, dim3(1), dim3(64), 0, st, (unsigned*)q, (int64_t)1, 0u);
+    return q;
+}
+
 int mea_lds(int Lmax) { return (int)align_up((size_t)(Lmax + 2) * 4 + 2 * (Lmax + 2) + 16, 16) + 2 * (Lmax + 2) * 4 + 16; }
 int vit_lds(int Lmax) { return cf_vit_plan(Lmax).total; }
 
 // ---------------------------------------------------------------- stages
-// A bucket whose tuned width is one wave (64 threads) runs packed: kPack
+// A bucket whose tuned width is one wave (64 threads) runs packed: several
 // sequences per workgroup, one wave each, sharing the staged parameter
-// tables.  Wider buckets give the whole workgroup to one sequence.
-constexpr int kPack = 4;
-// sequences per workgroup of the packed inside / outside sweeps; fewer when
-// LDS is short.  Alone, at 10,000 x 200 nt (inside / outside ms): 1 per
-// workgroup 90.7 / 120.3, 2: 65.4 / 87.1, 4: 58.3 / 78.8, 8: 57.0 / 77.0 --
-// eight one-wave teams per workgroup keep 32 teams on a CU (LDS-limited)
-// against 24, and more resident folds hide more of the split reads' latency.
-// But the outside sweep shares the GPU with the sampler in a logZ + bpp +
-// samples run, and there its 512-thread, 40 KB workgroups found CU room less
-// often: 80.7 -> 104.7 ms and 58.2k -> 53.3k folds/s.  So the inside packs
-// eight and the outside four.
-constexpr int kPackIn = 8;
-constexpr int kPackOut = 4;
+// tables, and the workgroups take folds from a queue until it is empty.
+// Wider buckets give the whole workgroup to one sequence.  Sequences per
+// workgroup of the packed inside / outside sweeps (fewer when LDS is short),
+// alone, at 10,000 x 200 nt (inside / outside ms): 1 per workgroup
+// 90.7 / 120.3, 2: 65.4 / 87.1, 4: 58.3 / 78.8, 8: 57.0 / 77.0 --
+// default packing (cf_ctx::pack_in / pack_out): eight one-wave teams per
+// workgroup keep 32 teams on a CU (LDS-limited) against 24, and more resident
+// folds hide more of the split reads' latency.  But the outside sweep shares
+// the GPU with the sampler in a logZ + bpp + samples run, and there its
+// 512-thread, 40 KB workgroups found CU room less often: 80.7 -> 104.7 ms and
+// 58.2k -> 53.3k folds/s.  So the inside packs eight and the outside four.
 constexpr int kLanes = 4;  // streams a multi-bucket batch forks onto
 
 // Workgroup width of the kernels that have no packed layout (MEA, Viterbi,
@@ -147,25 +162,29 @@ int launch_inside(cf_batch* b, const Bucket& bk, hipStream_t st) {
     // ~26 KB of LDS at 200 nt, and more resident waves per CU beat the ring
     // on throughput (54 vs 87 ms for 3000 x 200 nt).
     const bool pack = bk.threads <= 64;
-    // packed: as many sequences per workgroup (up to kPackIn) as LDS allows
-    int G = pack ? kPackIn : 1;
+    // packed: as many sequences per workgroup (up to pack_in) as LDS allows
+    int G = pack ? c->pack_in : 1;
     CfLds plan = cf_lds_plan(bk.Lmax, false, 0, G);
     while (plan.total > c->lds_max && G > 1) plan = cf_lds_plan(bk.Lmax, false, 0, G >>= 1);
     if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the inside kernel's LDS plan");
     const int K = pack ? 1 : bk.split;
     if (K > 1) clear_xsync(b, bk, st);
-    if (pack)
-        hipLaunchKernelGGL((cf_inside_kernel<true, false>), dim3((nl + G - 1) / G), dim3(64 * G), plan.total, st, b->d_seqs,
+    if (pack) {
+        int* q = clear_queue(b, bk, 0, st);
+        const int grid = persistent_grid(c, cf_inside_kernel<true, false>, G, plan.total, nl);
+        hipLaunchKernelGGL((cf_inside_kernel<true, false>), dim3(grid), dim3(64 * G), plan.total, st, b->d_seqs,
                            b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
-                           b->FCX, bk.Lmax, 1, (unsigned*)nullptr, b->d_status);
+                           b->FCX, bk.Lmax, 1, (unsigned*)nullptr, b->d_status, q);
+    }
     else if (K > 1)
         hipLaunchKernelGGL((cf_inside_kernel<false, true>), dim3(cf_split_grid(nl, K)), dim3(bk.threads), plan.total,
                            st, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM,
-                           b->FM1, b->FM2, b->FCX, bk.Lmax, K, b->d_xsync + (size_t)bk.begin * 16, b->d_status);
+                           b->FM1, b->FM2, b->FCX, bk.Lmax, K, b->d_xsync + (size_t)bk.begin * 16, b->d_status,
+                           (int*)nullptr);
     else
         hipLaunchKernelGGL((cf_inside_kernel<false, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
                            b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->FM2,
-                           b->FCX, bk.Lmax, 1, (unsigned*)nullptr, b->d_status);
+                           b->FCX, bk.Lmax, 1, (unsigned*)nullptr, b->d_status, (int*)nullptr);
     CF_HIP(c, hipGetLastError(), "inside launch");
     return CF_OK;
 }
@@ -193,14 +212,16 @@ int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st) {
     if (nl <= 0) return CF_OK;
     if (bk.threads <= 64) {
         // packed: no room for a per-sequence FCo ring, the window is read from L2
-        int G = kPackOut;
+        int G = c->pack_out;
         CfLds plan = cf_lds_plan(bk.Lmax, false, 0, G);
         while (plan.total > c->lds_max && G > 1) plan = cf_lds_plan(bk.Lmax, false, 0, G >>= 1);
         if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
-        hipLaunchKernelGGL((cf_outside_kernel<false, true, false>), dim3((nl + G - 1) / G), dim3(64 * G),
+        int* q = clear_queue(b, bk, 1, st);
+        const int grid = persistent_grid(c, cf_outside_kernel<false, true, false>, G, plan.total, nl);
+        hipLaunchKernelGGL((cf_outside_kernel<false, true, false>), dim3(grid), dim3(64 * G),
                            plan.total, st, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32,
                            b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, 1,
-                           (unsigned*)nullptr, b->d_status);
+                           (unsigned*)nullptr, b->d_status, q);
         CF_HIP(c, hipGetLastError(), "outside launch");
         return CF_OK;
     }
@@ -213,7 +234,7 @@ int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st) {
         hipLaunchKernelGGL((cf_outside_kernel<false, false, true>), dim3(cf_split_grid(nl, K)), dim3(bk.threads),
                            plan.total, st, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32,
                            b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, K,
-                           b->d_xsync + (size_t)bk.begin * 16, b->d_status);
+                           b->d_xsync + (size_t)bk.begin * 16, b->d_status, (int*)nullptr);
         CF_HIP(c, hipGetLastError(), "outside launch");
         return CF_OK;
     }
@@ -224,11 +245,13 @@ int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st) {
     if (ring)
         hipLaunchKernelGGL((cf_outside_kernel<true, false, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
                            b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
-                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, 1, (unsigned*)nullptr, b->d_status);
+                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, 1, (unsigned*)nullptr, b->d_status,
+                           (int*)nullptr);
     else
         hipLaunchKernelGGL((cf_outside_kernel<false, false, false>), dim3(nl), dim3(bk.threads), plan.total, st, b->d_seqs,
                            b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32, b->FC, b->FM, b->FM1, b->F5,
-                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, 1, (unsigned*)nullptr, b->d_status);
+                           b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, 1, (unsigned*)nullptr, b->d_status,
+                           (int*)nullptr);
     CF_HIP(c, hipGetLastError(), "outside launch");
     return CF_OK;
 }
@@ -452,11 +475,14 @@ int enqueue_bucket(cf_batch* b, const Bucket& bk, unsigned need, hipStream_t st)
 // The pipelines a forked run spreads over the lane streams: every length
 // bucket, and a bucket of uneven lengths (longest >= 1.5x shortest) with at
 // least 2 * kPartMin sequences cut into up to kLanes consecutive parts.  In
-// such a bucket each stage ends with the longest folds still running on a
-// partly idle GPU; parts on different streams let the next stage of the
-// longest part start in that idle room (20,000 x U[30, 500] nt: 10,343 ->
-// 10,856 folds/s).  Equal lengths gain nothing from it (10,000 x 200 nt:
-// 190.0 -> 194.4 ms per step with four parts), so such buckets stay whole.
+// such a bucket each stage ends with its last folds still running on a
+// partly idle GPU (the packed sweeps' fold queue hands out the longest folds
+// first, so these are short ones, but the next stage waits for all of them);
+// parts on different streams let the next stage of the longest part start in
+// that idle room (20,000 x U[30, 500] nt, MEA at 3 gammas + Viterbi, one
+// packed bucket: 9,910 folds/s whole, 10,860 in four parts).  Equal lengths
+// gain nothing from it (10,000 x 200 nt: 190.0 -> 194.4 ms per step with four
+// parts), so such buckets stay whole.
 constexpr int kPartMin = 1024;
 std::vector<Bucket> fork_parts(const cf_batch* b) {
     std::vector<Bucket> parts;
@@ -585,6 +611,7 @@ size_t batch_staging_bytes(int n, int64_t C) {
         k.take<float>(kMaxGamma + n);
         k.take<int>(4);
         k.take<unsigned>((int64_t)16 * n);
+        k.take<int>((int64_t)2 * n);
     });
 }
 
@@ -748,6 +775,7 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
     b->d_gin = d.take<float>(kMaxGamma + n);
     b->d_status = d.take<int>(4);
     b->d_xsync = d.take<unsigned>((int64_t)16 * n);
+    b->d_queue = d.take<int>((int64_t)2 * n);
     for (auto& ev : b->ev) {
         if (hipEventCreate(&ev) != hipSuccess) {
             cf_batch_destroy(b);
@@ -865,6 +893,7 @@ struct NoAllocLayout {
     float* gin;
     int* status;
     unsigned* xsync;
+    int* queue;
     float *FC, *FM, *FM1, *FM2, *FCX, *F5, *F5o, *logz, *FCo, *FMo, *FM1o, *FM2o, *POST, *SC;
     int16_t* TB;
     uint8_t* mea_out;
@@ -881,6 +910,7 @@ NoAllocLayout noalloc_layout(char* base, int n, int64_t C, int64_t T, int64_t X,
     l.gin = k.take<float>(kMaxGamma + n);
     l.status = k.take<int>(4);
     l.xsync = k.take<unsigned>((int64_t)16 * n);
+    l.queue = k.take<int>((int64_t)2 * n);
     l.FC = k.take<float>(T);
     l.FM = k.take<float>(T);
     l.FM1 = k.take<float>(T);
@@ -1022,6 +1052,14 @@ int cf_set_split(cf_ctx* c, int min_length, int wgs_per_fold) {
         return fail(c, CF_EINVAL, "split: min_length >= 0 and 1..16 workgroups per fold");
     c->split_min = wgs_per_fold > 1 ? min_length : 0;
     c->split_k = wgs_per_fold;
+    return CF_OK;
+}
+
+int cf_set_packing(cf_ctx* c, int teams_inside, int teams_outside) {
+    if (!c || teams_inside < 1 || teams_inside > 16 || teams_outside < 1 || teams_outside > 16)
+        return fail(c, CF_EINVAL, "packing: 1..16 teams per workgroup");
+    c->pack_in = teams_inside;
+    c->pack_out = teams_outside;
     return CF_OK;
 }
 
@@ -1299,6 +1337,7 @@ int cf_fold_noalloc(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, 
     b.d_gin = l.gin;
     b.d_status = l.status;
     b.d_xsync = l.xsync;
+    b.d_queue = l.queue;
     b.FC = l.FC; b.FM = l.FM; b.FM1 = l.FM1; b.FM2 = l.FM2; b.FCX = l.FCX;
     b.F5 = l.F5; b.F5o = l.F5o; b.logz = l.logz;
     b.FCo = l.FCo; b.FMo = l.FMo; b.FM1o = l.FM1o; b.FM2o = l.FM2o; b.POST = l.POST;

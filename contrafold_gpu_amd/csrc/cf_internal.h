@@ -71,6 +71,9 @@ struct cf_ctx {
     // split teams: folds of at least split_min nt in a wide bucket run on
     // split_k workgroups of 256 threads each (0 = off); see cf_set_split
     int split_min = 0, split_k = 1;
+    // one-wave teams per workgroup of the packed inside and outside sweeps
+    // (fewer when LDS is short); see cf_set_packing
+    int pack_in = 8, pack_out = 4;
     int lds_max = 65536;
     int n_cu = 256;
     cfi::DevPool pool;
@@ -110,6 +113,7 @@ struct cf_batch {
     int* d_status = nullptr;    // per-run kernel status bits (1: sampler stack overflow, 2: split team not
                                 // co-resident), zeroed by each run
     unsigned* d_xsync = nullptr;  // split-team barrier counters: 16 words per sequence (by launch order)
+    int* d_queue = nullptr;       // fold queues of persistent packed launches: 2 per sequence (by launch order)
     float *FC = nullptr, *FM = nullptr, *FM1 = nullptr, *FM2 = nullptr, *FCX = nullptr, *F5 = nullptr,
           *F5o = nullptr, *logz = nullptr;
     // outside

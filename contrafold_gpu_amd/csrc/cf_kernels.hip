@@ -233,6 +233,16 @@ __device__ __forceinline__ CfTeam cf_team(int G, int K = 1) {
 // Launch size of a split launch over n folds: whole blocks of 8 folds.
 __host__ __device__ inline int cf_split_grid(int n, int K) { return ((n + 7) / 8) * 8 * K; }
 
+// Persistent packed launches: a grid sized to what the device holds at once,
+// whose one-wave teams take folds from a queue in launch order (longest
+// first) until it is empty -- a team that finishes early takes the next fold,
+// so uneven lengths need no cut into stream parts.  Lane 0 takes the ticket.
+__device__ __forceinline__ int cf_next_fold(int* queue) {
+    int v = 0;
+    if ((threadIdx.x & (WAVE - 1)) == 0) v = atomicAdd(queue, 1);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Barrier of a split team: every workgroup of the fold arrives on the fold's
 // counter and waits until all K have arrived for this generation.  Each wave
 // first waits for its own loads and stores to complete; thread 0 then
@@ -309,7 +319,7 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_inside_kernel(const CfSeq
                                                         float* __restrict__ FM, float* __restrict__ FM1,
                                                         float* __restrict__ FM2, float* __restrict__ FCX, int Lmax,
                                                         int K, unsigned* __restrict__ xsync,
-                                                        int* __restrict__ status) {
+                                                        int* __restrict__ status, int* __restrict__ queue) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int xabort;
     const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
@@ -319,13 +329,17 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_inside_kernel(const CfSeq
     cf_stage_params(S, P);
     if (threadIdx.x == 0) xabort = 0;
     __syncthreads();
-    if (tm.li >= nseq) return;
-    const CfSeq sd = seqs[order[tm.li]];
+    // queue (packed teams only): take folds until the queue is empty
+    const bool persistent = PACK && queue != nullptr;
+    for (int li = persistent ? cf_next_fold(queue) : tm.li; li < nseq;
+         li = persistent ? cf_next_fold(queue) : nseq) {
+    if (persistent) cf_team_sync<PACK>();  // the team's LDS area is reused fold after fold
+    const CfSeq sd = seqs[order[li]];
     const int L = sd.L;
     const int tid = tm.tid, NT = tm.NT;
     cf_stage_seq<PACK>(S, codes, masks, sd, tm.ltid, tm.lNT);
     // team-wide sync (every part of a split team) and workgroup-local sync
-    CfXBar xb{tm.K > 1 ? xsync + (size_t)tm.li * 16 : nullptr, status, &xabort, 0u, tm.K};
+    CfXBar xb{tm.K > 1 ? xsync + (size_t)li * 16 : nullptr, status, &xabort, 0u, tm.K};
     auto sync_team = [&]() {
         if (tm.K > 1) cf_xbar(xb);
         else cf_team_sync<PACK>();
@@ -607,6 +621,7 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_inside_kernel(const CfSeq
             sync_team();
         }
     }
+    }  // folds of this team
 }
 
 // ================================================================== exterior
@@ -791,7 +806,8 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
     const uint8_t* __restrict__ masks, const CfParams<float>* __restrict__ P, const float* __restrict__ FC,
     const float* __restrict__ FM, const float* __restrict__ FM1, const float* __restrict__ F5,
     const float* __restrict__ F5o, float* __restrict__ FCo, float* __restrict__ FMo, float* __restrict__ FM1o,
-    float* __restrict__ FM2o, int Lmax, int K, unsigned* __restrict__ xsync, int* __restrict__ status) {
+    float* __restrict__ FM2o, int Lmax, int K, unsigned* __restrict__ xsync, int* __restrict__ status,
+    int* __restrict__ queue) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int xabort;
     const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
@@ -802,13 +818,16 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
     cf_stage_params(S, P);
     if (threadIdx.x == 0) xabort = 0;
     __syncthreads();
-    if (tm.li >= nseq) return;
-    const CfSeq sd = seqs[order[tm.li]];
+    const bool persistent = PACK && queue != nullptr;
+    for (int li = persistent ? cf_next_fold(queue) : tm.li; li < nseq;
+         li = persistent ? cf_next_fold(queue) : nseq) {
+    if (persistent) cf_team_sync<PACK>();  // the team's LDS area is reused fold after fold
+    const CfSeq sd = seqs[order[li]];
     const int L = sd.L;
     const int tid = tm.tid, NT = tm.NT;
     const int lane = tid & (WAVE - 1);
     cf_stage_seq<PACK>(S, codes, masks, sd, tm.ltid, tm.lNT);
-    CfXBar xb{tm.K > 1 ? xsync + (size_t)tm.li * 16 : nullptr, status, &xabort, 0u, tm.K};
+    CfXBar xb{tm.K > 1 ? xsync + (size_t)li * 16 : nullptr, status, &xabort, 0u, tm.K};
     auto sync_team = [&]() {
         if (tm.K > 1) cf_xbar(xb);
         else cf_team_sync<PACK>();
@@ -1198,6 +1217,7 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
             sync_team();
         }
     }
+    }  // folds of this team
 }
 
 // ================================================================ posterior
@@ -2673,30 +2693,30 @@ __global__ void __launch_bounds__(256) cf_grad_reduce_kernel(const double* __res
 // ------------------------------------------------------------ explicit inits
 template __global__ void cf_inside_kernel<false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                  const CfParams<float>*, float*, float*, float*, float*, float*, int,
-                                                 int, unsigned*, int*);
+                                                 int, unsigned*, int*, int*);
 template __global__ void cf_inside_kernel<true, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                 const CfParams<float>*, float*, float*, float*, float*, float*, int,
-                                                int, unsigned*, int*);
+                                                int, unsigned*, int*, int*);
 template __global__ void cf_outside_kernel<true, false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                         const CfParams<float>*, const float*, const float*, const float*,
                                                         const float*, const float*, float*, float*, float*, float*,
-                                                        int, int, unsigned*, int*);
+                                                        int, int, unsigned*, int*, int*);
 template __global__ void cf_outside_kernel<false, false, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                          const CfParams<float>*, const float*, const float*,
                                                          const float*, const float*, const float*, float*, float*,
-                                                         float*, float*, int, int, unsigned*, int*);
+                                                         float*, float*, int, int, unsigned*, int*, int*);
 template __global__ void cf_outside_kernel<false, true, false>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                         const CfParams<float>*, const float*, const float*, const float*,
                                                         const float*, const float*, float*, float*, float*, float*,
-                                                        int, int, unsigned*, int*);
+                                                        int, int, unsigned*, int*, int*);
 template __global__ void cf_inside_kernel<false, true>(const CfSeq*, const int*, int, const uint8_t*, const uint8_t*,
                                                        const CfParams<float>*, float*, float*, float*, float*, float*,
-                                                       int, int, unsigned*, int*);
+                                                       int, int, unsigned*, int*, int*);
 template __global__ void cf_outside_kernel<false, false, true>(const CfSeq*, const int*, int, const uint8_t*,
                                                                const uint8_t*, const CfParams<float>*, const float*,
                                                                const float*, const float*, const float*, const float*,
                                                                float*, float*, float*, float*, int, int, unsigned*,
-                                                               int*);
+                                                               int*, int*);
 template __global__ void cf_viterbi_kernel<true>(const CfSeq*, const int*, const uint8_t*, const uint8_t*,
                                                  const CfParams<double>*, double*, double*, double*, double*, uint8_t*,
                                                  const int64_t*, int);

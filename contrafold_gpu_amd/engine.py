@@ -137,6 +137,13 @@ class Engine:
         with self.lock:
             check(self.lib.cf_set_split(self.ctx, int(min_length), int(wgs_per_fold)), self.ctx, "cf_set_split")
 
+    def set_packing(self, teams_inside, teams_outside):
+        """One-wave teams per workgroup of the packed inside and outside
+        sweeps (cf_set_packing, 1..16 each); results do not depend on it."""
+        with self.lock:
+            check(self.lib.cf_set_packing(self.ctx, int(teams_inside), int(teams_outside)), self.ctx,
+                  "cf_set_packing")
+
     def batch(self, seqs, masks=None, keys=None, sampling=True):
         return Batch(self, seqs, masks, keys, sampling)
 

@@ -77,10 +77,18 @@ int cf_set_params(cf_ctx *ctx, const double *tables);
 
 /* Length buckets: sequences with L <= edges[k] (and above edges[k-1]) run in
  * one launch per kernel with team width threads[k] (a multiple of 64 in
- * 64..1024): 64 packs four sequences into a workgroup, one wave each; wider
- * teams give a sequence the whole workgroup (lower latency for long folds).
- * Defaults are set at cf_create; the autotuner overrides them. */
+ * 64..1024): 64 packs several sequences into a workgroup, one wave each
+ * (cf_set_packing); wider teams give a sequence the whole workgroup (lower
+ * latency for long folds).  Defaults are set at cf_create; the autotuner
+ * overrides them. */
 int cf_set_tuning(cf_ctx *ctx, const int *edges, const int *threads, int n_buckets);
+
+/* Packing (no counterpart in the reference, which launches one thread block
+ * per sequence, gpu.py:508): one-wave teams per workgroup, 1..16, of the
+ * packed (width 64) inside and outside sweeps; fewer are used when LDS is
+ * short.  Results do not depend on it.  Defaults 8 and 4; the autotuner
+ * overrides them. */
+int cf_set_packing(cf_ctx *ctx, int teams_inside, int teams_outside);
 
 /* Split teams (no counterpart in the reference, whose kernel gives a fold
  * one thread block, gpu.py:55-184, 508): in a wide bucket (team width > 64)

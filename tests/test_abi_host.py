@@ -219,7 +219,8 @@ def test_serving_steady_rate():
 
 def test_autotune_with_a_fake_engine(tmp_path, monkeypatch):
     """Host logic of the tuner: it times every candidate on at least a third of
-    the workload (up to 8,000 sequences), installs the fastest and caches it."""
+    the workload (up to 8,000 sequences), then the packings on the fastest one
+    when it has a packed bucket, installs the fastest and caches it."""
     from contrafold_gpu_amd import autotune
     monkeypatch.setenv("CF_TUNE_CACHE", str(tmp_path / "tune.json"))
 
@@ -233,29 +234,40 @@ def test_autotune_with_a_fake_engine(tmp_path, monkeypatch):
             return False
         def run(self, flags, **kw):
             pass
-        def last_ms(self):  # the candidate with a 256-wide last bucket is fastest
-            return 5.0 if self.eng.cur[1][-1] == 256 else 9.0
+        def last_ms(self):  # a 256-wide last bucket is fastest, and packing (4, 4) on it
+            return (5.0 if self.eng.cur[1][-1] == 256 else 9.0) - (1.0 if self.eng.pack == (4, 4) else 0.0)
 
     class FakeEngine:
         def __init__(self):
-            self.sizes, self.cur, self.installed = [], None, []
+            self.sizes, self.cur, self.installed, self.pack = [], None, [], None
         def set_tuning(self, edges, threads):
             self.cur = (list(edges), list(threads))
             self.installed.append(self.cur)
+        def set_packing(self, inside, outside):
+            self.pack = (inside, outside)
         def batch(self, seqs, masks=None):
             return FakeBatch(self, seqs)
 
     eng = FakeEngine()
     seqs = ["ACGU" * 10] * 9000
     res = autotune.tune(eng, seqs, sample=1500, repeats=1)
-    assert not res["cached"] and len(res["tried"]) == len(autotune.CANDIDATES)
+    n_cand, n_pack = len(autotune.CANDIDATES), len(autotune.PACKINGS)
+    assert not res["cached"] and len(res["tried"]) == n_cand + n_pack - 1
     assert set(eng.sizes) == {3000}                      # a third of 9,000, not 1,500
     assert res["threads"][-1] == 256 and eng.cur == (res["edges"], res["threads"])
+    assert 64 in res["threads"] and res["packing"] == [4, 4] and eng.pack == (4, 4) and res["ms"] == 4.0
+    assert all(t["packing"] == list(autotune.PACKINGS[0]) for t in res["tried"][:n_cand])
+    eng.pack = None
     again = autotune.tune(eng, seqs, sample=1500, repeats=1)
-    assert again["cached"] and again["threads"] == res["threads"]
+    assert again["cached"] and again["threads"] == res["threads"] and eng.pack == (4, 4)
     fe = FakeEngine()
     big = autotune.tune(fe, ["ACGU" * 30] * 60000, sample=1500, repeats=1, use_cache=False)
-    assert big["ms"] == 5.0 and set(fe.sizes) == {8000}  # capped at 8,000
+    assert big["ms"] == 4.0 and set(fe.sizes) == {8000}  # capped at 8,000
+    # no packed bucket among the candidates: no packing phase, default packing
+    fw = FakeEngine()
+    wide = autotune.tune(fw, seqs, sample=1500, repeats=1, use_cache=False,
+                         candidates=[([autotune.BIG], [128]), ([autotune.BIG], [256])])
+    assert len(wide["tried"]) == 2 and wide["packing"] == list(autotune.PACKINGS[0]) and wide["ms"] == 5.0
 
 
 def test_one_hip_runtime_per_process():

@@ -398,12 +398,23 @@ def test_autotune_installs_fastest(engine, P, tmp_path, monkeypatch):
     rng = np.random.default_rng(17)
     seqs = [rand_seq(rng, int(rng.integers(30, 300))) for _ in range(60)]
     cands = autotune.CANDIDATES[:3]
-    res = autotune.tune(engine, seqs, flags=CF_BPP, sample=60, repeats=1, candidates=cands)
-    assert not res["cached"] and len(res["tried"]) == 3
+    packs = [(8, 4), (2, 16)]
+    with engine.batch(seqs) as b:
+        b.run(CF_BPP)
+        ref = [b.bpp(k) for k in range(len(seqs))]
+    res = autotune.tune(engine, seqs, flags=CF_BPP, sample=60, repeats=1, candidates=cands, packings=packs)
+    assert not res["cached"] and len(res["tried"]) == 4  # every candidate has a packed bucket
     assert res["ms"] == min(t["ms"] for t in res["tried"])
-    again = autotune.tune(engine, seqs, flags=CF_BPP, sample=60, repeats=1, candidates=cands)
-    assert again["cached"] and again["edges"] == res["edges"]
+    again = autotune.tune(engine, seqs, flags=CF_BPP, sample=60, repeats=1, candidates=cands, packings=packs)
+    assert again["cached"] and again["edges"] == res["edges"] and again["packing"] == res["packing"]
+    # results do not depend on the tuning or the packing
+    for pk in packs:
+        engine.set_packing(*pk)
+        with engine.batch(seqs) as b:
+            b.run(CF_BPP)
+            assert all(np.array_equal(b.bpp(k), ref[k]) for k in range(len(seqs)))
     engine.set_tuning(*autotune.DEFAULT)
+    engine.set_packing(*autotune.PACKINGS[0])
 
 
 def test_cli_modes(P, tmp_path, capsys):
