@@ -848,11 +848,16 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
     float* __restrict__ fm2o_t = FM2o + sd.toff;
     const float4* __restrict__ C = S.lse;
 
-    // FMo[a][b] += FMo[a][b+1] + mu: the last source of a completed FMo cell
-    auto fmo_finish = [&](int d) {
+    // FMo[a][b] += FMo[a][b+1] + mu: the last source of a completed FMo cell.
+    // Each thread finishes the cells it took in phase A (the same top-down
+    // strips of na columns), so a split team needs no cross-workgroup barrier
+    // between phase A and finishing diagonal s in phase B.
+    auto fmo_finish = [&](int d, int na) {
         const int ncell = L + 1 - d;
         const int db = S.dtab[d];
-        for (int a = tid; a < ncell; a += NT) {
+        for (int c0 = __builtin_amdgcn_readfirstlane(tid - lane); c0 < na; c0 += NT) {
+            const int a = na - WAVE - c0 + lane;
+            if (a < 0 || a >= ncell) continue;
             const int b = a + d;
             const bool dom = (a > 0) && (a + 2 <= b) && (b < L);
             if (dom && b + 1 < L) {
@@ -963,10 +968,14 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
         // A lane takes column a of both diagonals: cells (a, a+s) and
         // (a, a+s-1) fold FM2o of their own rows against the same FM1[i'][a],
         // so that column is read once for two cells.
+        // A cell's split chain is as long as its column a, and a wave loops as
+        // long as its top lane: the waves take 64-column strips from the top
+        // down, so the partial strip (na not a multiple of 64) is the one with
+        // the shortest chains.
         const int na = max(n0, n1);
         for (int c0 = __builtin_amdgcn_readfirstlane(tid - lane); c0 < na; c0 += NT) {
-            const int a = c0 + lane;
-            const bool live0 = a < n0, live1 = a < n1;
+            const int a = na - WAVE - c0 + lane;
+            const bool live0 = a >= 0 && a < n0, live1 = a >= 0 && a < n1;
             const bool dom0 = live0 && (a > 0) && (s >= 2) && (a + s < L);
             const bool dom1 = live1 && (a > 0) && (s >= 3) && (a + s - 1 < L);
             float f0 = CF_NEG, f1 = CF_NEG;
@@ -975,7 +984,7 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
             // diagonals s + u, s - 1 + u (FM2o) and u (FM1) step down with u;
             // lanes outside a domain read cell 0 of the diagonal and fold NEG,
             // which leaves the sum unchanged.
-            const int amax = min(c0 + WAVE - 1, na - 1);
+            const int amax = na - 1 - c0;
             int u = amax - 1;
             int ox0 = (int)cf_dbase(L, s + u), ox1 = (int)cf_dbase(L, s - 1 + u), oy = (int)cf_dbase(L, u);
             {
@@ -1205,10 +1214,10 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
             fco_t[db + a] = acc;
             if (RING) ring[(d % CF_RING) * RS + a] = acc;
         }
-        fmo_finish(s);
+        fmo_finish(s, na);
         sync_team();
         // ---- C: FMo of s-1 completed; FM2o, FM1o of s
-        if (s >= 1) fmo_finish(s - 1);
+        if (s >= 1) fmo_finish(s - 1, na);
         fm12o(s);
         sync_team();
         // ---- D: FM2o, FM1o of s-1

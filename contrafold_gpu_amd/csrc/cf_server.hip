@@ -2,7 +2,8 @@
 // reference, which folds synchronously per call, gpu.py:480-583).
 //
 // Requests wait in a FIFO.  Whenever one of the server's streams is idle, the
-// scheduler takes the head request and every later pending request of the
+// scheduler picks a head request (the oldest of its group with the highest
+// response ratio, see launch) and takes every later pending request of the
 // same kind (MEA / sample / bpp / logZ, same gamma and sample count) and the
 // same length class, up to max_batch, as long as their device table bytes
 // (cf_fold_bytes, O(L^2)) fit the HBM budget next to the batches already in
@@ -166,6 +167,10 @@ int serve_width(int Lmax) { return Lmax <= 100 ? 128 : Lmax <= 250 ? 256 : Lmax 
 // Folds of length class >= kLongClass take long; at most half of the slots
 // may hold them, so short requests always find a slot.
 constexpr int kLongClass = 3;
+// Typical device time (us) of a small batch of each length class on a loaded
+// GPU (measured run time p50 of the serving mix: <= 100 nt 10 ms, 101-200 nt
+// 45 ms, 201-400 nt 60 ms, 401-700 nt 190 ms, longer 250 ms).
+constexpr double kClassUs[6] = {10e3, 40e3, 60e3, 150e3, 200e3, 250e3};
 
 int launch(cf_server* s, Slot& sl) {
     cf_ctx* c = s->ctx;
@@ -173,19 +178,39 @@ int launch(cf_server* s, Slot& sl) {
     s->max_pending = std::max(s->max_pending, (double)s->pending.size());
     while (!s->pending.empty() && s->reqs[s->pending.front()].state != 0) s->pending.pop_front();
     if (s->pending.empty()) return 0;
-    // the oldest request decides the batch kind and length class, unless it is
-    // long and the long folds already hold their share of the slots
+    // Which group (kind, gamma, sample count, length class) goes next: the one
+    // whose oldest request has the highest response ratio, its wait over its
+    // class's typical fold time, so short folds do not queue behind a long
+    // batch's turn and a long request's priority still grows while it waits.
+    // Long classes are skipped while they hold their share of the slots.
     int long_busy = 0;
     for (const auto& o : s->slots)
         if (o.batch && o.long_class) ++long_busy;
     const bool long_ok = long_busy < (int)s->slots.size() / 2;
+    const double now = s->now_us();
     int64_t head_id = -1;
+    double best = -1.0;
+    std::vector<int64_t> seen;  // the oldest request of each group met so far
     for (int64_t id : s->pending) {
         const Req& r = s->reqs[id];
         if (r.state != 0) continue;
-        if (long_ok || length_class((int)r.seq.size()) < kLongClass) {
+        const int lc = length_class((int)r.seq.size());
+        if (!long_ok && lc >= kLongClass) continue;
+        bool older = false;
+        for (int64_t o : seen) {
+            const Req& q = s->reqs[o];
+            if (q.kind == r.kind && q.gamma == r.gamma && q.n_samples == r.n_samples &&
+                length_class((int)q.seq.size()) == lc) {
+                older = true;
+                break;
+            }
+        }
+        if (older) continue;
+        seen.push_back(id);
+        const double ratio = (now - r.arrival_us + kClassUs[lc]) / kClassUs[lc];
+        if (ratio > best) {
+            best = ratio;
             head_id = id;
-            break;
         }
     }
     if (head_id < 0) return 0;

@@ -240,3 +240,16 @@ def test_mea_exact_tie_split(oracle):
     pair, obj = oracle.mea_decode32(POST, "GCGC", 1.0)
     assert obj == np.float32(1.0)
     assert oracle.dot_bracket(pair, 4) == "(..)"
+
+
+def test_printed_partition_coefficient(oracle, P):
+    """gpu_contrafold/reference.py:32 shows the CONTRAfold binary's output line
+    'Log partition coefficient for "...": 3.05664' (six significant digits; the
+    sequence is not named there, the README's example GGGGAAAACCCC is the
+    natural candidate).  The Fast_LogExpPlusOne path that the binary and the
+    FP32 engine share agrees to the printed digits (3.0566349 in FP32: within
+    the print's half unit, 5e-6, plus FP32 rounding of the sums); the exact
+    FP64 inside (3.05661) is within the approximation's error of it."""
+    z32, _ = oracle.posterior32("GGGGAAAACCCC", P, None)
+    assert abs(float(z32) - 3.05664) <= 1e-5
+    assert abs(oracle.inside64("GGGGAAAACCCC", P) - 3.05664) <= 5e-4
