@@ -37,6 +37,7 @@ CANDIDATES = [
 DEFAULT = CANDIDATES[0]  # the library's built-in tuning
 # (inside, outside) one-wave teams per workgroup; the first is the default
 PACKINGS = [(8, 4), (4, 4), (8, 8), (16, 4), (4, 2), (2, 2)]
+PACKING_MARGIN = 0.03  # a packing other than the default must be this much faster
 
 
 def cache_path():
@@ -126,13 +127,18 @@ def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, samp
     edges, threads = best[1], best[2]
     engine.set_tuning(edges, threads)
     if 64 in threads:  # a packed bucket: try the other packings on the chosen buckets
+        # Another packing must win clearly: the sample fills the GPU less than
+        # the whole batch, and within a few percent the default holds up better
+        # on the batch (20,000 x U[30, 500] nt: (4, 2) 0.5% faster on the
+        # sample, 5% slower on the batch).
+        bar = best[0] * (1.0 - PACKING_MARGIN)
         for pk in packings[1:]:
             engine.set_packing(*pk)
             ms = timed()
             results.append((ms, edges, threads, pk))
             if log:
                 log(f"[tune] packing={pk}: {ms:.2f} ms")
-            if ms < best[0]:
+            if ms < bar and ms < best[0]:
                 best = (ms, edges, threads, pk)
     ms, edges, threads, packing = best
     engine.set_packing(*packing)

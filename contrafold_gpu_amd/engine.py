@@ -50,6 +50,13 @@ def encode_batch(seqs, masks=None):
     if masks is not None:
         if len(masks) != len(seqs):
             raise ValueError(f"forced_list has {len(masks)} masks for {len(seqs)} sequences")
+        if all(type(m) is np.ndarray and m.ndim == 1 for m in masks):
+            # the common case, one array per sequence of its own length: one
+            # concatenation instead of a copy per sequence
+            sizes = np.fromiter((m.size for m in masks), np.int64, len(masks))
+            if (sizes == lens).all():
+                mbuf = (np.concatenate(masks) != 0).view(np.uint8) if len(masks) else None
+                return buf, offs, mbuf, lens
         if any(m is not None for m in masks):
             mb = np.zeros(int(offs[-1]), np.uint8)
             for k, m in enumerate(masks):

@@ -99,6 +99,25 @@ def test_encode_batch_masks_and_errors():
     assert k1[0] == k2[1] and k1[2] != k1[0]          # content keys, repeats distinct
 
 
+def test_encode_batch_array_masks_fast_path():
+    """Masks given as one 1-D array per sequence of its own length take one
+    concatenation; the bytes equal the per-sequence path's (lists of the same
+    values), whatever the integer dtype and nonzero values."""
+    from contrafold_gpu_amd.engine import encode_batch
+    rng = np.random.default_rng(3)
+    seqs = ["ACGU" * int(n) for n in rng.integers(0, 30, 40)]
+    arrs = [(rng.random(len(q)) < 0.2).astype(dt) * v
+            for q, dt, v in zip(seqs, [np.int8, np.int64, np.uint8, np.int32] * 10, [1, 3, 255, -2] * 10)]
+    fast = encode_batch(seqs, arrs)[2]
+    slow = encode_batch(seqs, [list(a) for a in arrs])[2]
+    assert fast.dtype == np.uint8 and (fast == slow).all() and set(np.unique(fast)) <= {0, 1}
+    # a longer array is truncated (per-sequence path), a shorter one refused
+    long = [np.ones(len(q) + 2, np.int8) for q in seqs[:3]]
+    assert encode_batch(seqs[:3], long)[2].sum() == sum(len(q) for q in seqs[:3])
+    with pytest.raises(ValueError):
+        encode_batch(["ACGU"], [np.ones(2, np.int8)])
+
+
 def test_encode_batch_one_join_matches_per_sequence():
     """String batches are encoded with one join: the bytes and lengths equal
     per-sequence ASCII encoding with replacement (non-ASCII characters become
@@ -263,6 +282,14 @@ def test_autotune_with_a_fake_engine(tmp_path, monkeypatch):
     fe = FakeEngine()
     big = autotune.tune(fe, ["ACGU" * 30] * 60000, sample=1500, repeats=1, use_cache=False)
     assert big["ms"] == 4.0 and set(fe.sizes) == {8000}  # capped at 8,000
+    # a packing within the margin of the default does not replace it
+    fm = FakeEngine()
+    fm_batch_ms = FakeBatch.last_ms
+    FakeBatch.last_ms = lambda self: (5.0 if self.eng.cur[1][-1] == 256 else 9.0) - (
+        0.1 if self.eng.pack == (4, 4) else 0.0)
+    near = autotune.tune(fm, seqs, sample=1500, repeats=1, use_cache=False)
+    FakeBatch.last_ms = fm_batch_ms
+    assert near["packing"] == list(autotune.PACKINGS[0]) and near["ms"] == 5.0
     # no packed bucket among the candidates: no packing phase, default packing
     fw = FakeEngine()
     wide = autotune.tune(fw, seqs, sample=1500, repeats=1, use_cache=False,
