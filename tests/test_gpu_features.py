@@ -404,7 +404,13 @@ def test_autotune_installs_fastest(engine, P, tmp_path, monkeypatch):
         ref = [b.bpp(k) for k in range(len(seqs))]
     res = autotune.tune(engine, seqs, flags=CF_BPP, sample=60, repeats=1, candidates=cands, packings=packs)
     assert not res["cached"] and len(res["tried"]) == 4  # every candidate has a packed bucket
-    assert res["ms"] == min(t["ms"] for t in res["tried"])
+    # the fastest candidate at the default packing, unless the other packing is
+    # faster by the tuner's margin
+    base = min(t["ms"] for t in res["tried"] if t["packing"] == list(packs[0]))
+    if res["packing"] == list(packs[0]):
+        assert res["ms"] == base
+    else:
+        assert res["ms"] < base * (1.0 - autotune.PACKING_MARGIN)
     again = autotune.tune(engine, seqs, flags=CF_BPP, sample=60, repeats=1, candidates=cands, packings=packs)
     assert again["cached"] and again["edges"] == res["edges"] and again["packing"] == res["packing"]
     # results do not depend on the tuning or the packing
