@@ -117,12 +117,11 @@ int persistent_grid(cf_ctx* c, Kern kern, int G, size_t lds, int nfold) {
     return (int)std::max<int64_t>(1, std::min<int64_t>((nfold + G - 1) / G, (int64_t)per_cu * c->n_cu));
 }
 int* clear_queue(cf_batch* b, const cf_batch::Bucket& bk, int stage, hipStream_t st) {
-    int* q = b->d_queue + (size_t)bk.begin * 2 + stage;
+    int* q = b->d_queue + (size_t)bk.begin * 4 + stage;
     hipLaunchKernelGGL(cf_fill_u32, dim3(1), dim3(64), 0, st, (unsigned*)q, (int64_t)1, 0u);
     return q;
 }
 
-int mea_lds(int Lmax) { return (int)align_up((size_t)(Lmax + 2) * 4 + 2 * (Lmax + 2) + 16, 16) + 2 * (Lmax + 2) * 4 + 16; }
 int vit_lds(int Lmax) { return cf_vit_plan(Lmax).total; }
 
 // ---------------------------------------------------------------- stages
@@ -285,14 +284,27 @@ int launch_mea(cf_batch* b, const Bucket& bk, hipStream_t st) {
     cf_ctx* c = b->ctx;
     const int nl = bk.end - bk.begin;
     if (nl <= 0) return CF_OK;
-    const int lds = mea_lds(bk.Lmax);
-    if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the MEA kernel");
-    // each cell's split is a serial max chain: long folds get 512 threads so a
-    // diagonal's cells run in one pass (389 vs 440 ms for 20k x U[30,500] x 3 gammas)
-    const int threads = bk.Lmax > 256 ? 512 : solo_threads(bk);
-    hipLaunchKernelGGL(cf_mea_kernel, dim3(nl * b->n_gamma), dim3(threads), lds, st, b->d_seqs,
-                       b->d_order + bk.begin, nl, b->d_codes, b->d_masks, b->POST, b->d_gin, b->SC, b->TB, b->total_T,
-                       b->mea_out, b->d_charoff, b->total_chars, bk.Lmax);
+    if (bk.threads <= 64) {
+        // packed: one wave per (fold, gamma), eight per workgroup (fewer when
+        // LDS is short), taking items from the bucket's MEA queue
+        const int team = cf_mea_plan(bk.Lmax, 1).team;
+        int G = 8;
+        while (G > 1 && G * team > c->lds_max) G >>= 1;
+        if (team > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the MEA kernel");
+        int* q = clear_queue(b, bk, 2, st);
+        const int grid = persistent_grid(c, cf_mea_kernel<true>, G, (size_t)G * team, nl * b->n_gamma);
+        hipLaunchKernelGGL(cf_mea_kernel<true>, dim3(grid), dim3(64 * G), (size_t)G * team, st, b->d_seqs,
+                           b->d_order + bk.begin, nl, b->n_gamma, b->d_codes, b->d_masks, b->POST, b->d_gin, b->SC,
+                           b->total_M, b->mea_out, b->d_charoff, b->total_chars, bk.Lmax, q);
+    } else {
+        // wide: the workgroup (up to 8 waves) decodes one (fold, gamma)
+        const int nw = std::min(8, bk.threads / 64);
+        const int lds = cf_mea_plan(bk.Lmax, nw).team;
+        if (lds > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the MEA kernel");
+        hipLaunchKernelGGL(cf_mea_kernel<false>, dim3(nl * b->n_gamma), dim3(64 * nw), lds, st, b->d_seqs,
+                           b->d_order + bk.begin, nl, b->n_gamma, b->d_codes, b->d_masks, b->POST, b->d_gin, b->SC,
+                           b->total_M, b->mea_out, b->d_charoff, b->total_chars, bk.Lmax, (int*)nullptr);
+    }
     CF_HIP(c, hipGetLastError(), "mea launch");
     return CF_OK;
 }
@@ -384,14 +396,12 @@ int ensure_buffers(cf_batch* b, unsigned flags, int n_gamma, int n_samples) {
         if (n_gamma > b->mea_cap) {
             region_put(c, b->r_mea);
             auto lay = [&](Carver& k) {
-                k.take<float>((int64_t)n_gamma * b->total_T);
-                k.take<int16_t>((int64_t)n_gamma * b->total_T);
+                k.take<float>((int64_t)n_gamma * b->total_M);
                 k.take<uint8_t>((int64_t)n_gamma * b->total_chars);
             };
             if ((rc = region_get(c, b->r_mea, measure(lay), &b->buf_gen))) return rc;
             Carver k{(char*)b->r_mea.p};
-            b->SC = k.take<float>((int64_t)n_gamma * b->total_T);
-            b->TB = k.take<int16_t>((int64_t)n_gamma * b->total_T);
+            b->SC = k.take<float>((int64_t)n_gamma * b->total_M);
             b->mea_out = k.take<uint8_t>((int64_t)n_gamma * b->total_chars);
             b->mea_cap = n_gamma;
         }
@@ -611,7 +621,7 @@ size_t batch_staging_bytes(int n, int64_t C) {
         k.take<float>(kMaxGamma + n);
         k.take<int>(4);
         k.take<unsigned>((int64_t)16 * n);
-        k.take<int>((int64_t)2 * n);
+        k.take<int>((int64_t)4 * n);
     });
 }
 
@@ -684,15 +694,17 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
         }
     }
     std::vector<CfSeq> sd(n);
-    int64_t C = 0, T = 0, X = 0;
+    int64_t C = 0, T = 0, X = 0, M = 0;
     for (int i = 0; i < n; ++i) {
         sd[i].L = b->L[i];
         sd[i].coff = (int32_t)C;
         sd[i].toff = T;
         sd[i].xoff = X;
+        sd[i].moff = M;
         C += b->L[i] + 2;
         T += cf_tri_size(b->L[i]);
         X += std::max<int64_t>(cf_x_size(b->L[i]), 1);
+        M += cf_mea_cells(b->L[i]);
     }
     if (C > INT32_MAX) {
         delete b;
@@ -701,6 +713,7 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
     b->total_C = C;
     b->total_T = T;
     b->total_X = X;
+    b->total_M = M;
     const size_t in_bytes = batch_staging_bytes(n, C);
     int rc = region_get(c, b->r_in, in_bytes);
     if (rc) { delete b; return rc; }
@@ -775,7 +788,7 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
     b->d_gin = d.take<float>(kMaxGamma + n);
     b->d_status = d.take<int>(4);
     b->d_xsync = d.take<unsigned>((int64_t)16 * n);
-    b->d_queue = d.take<int>((int64_t)2 * n);
+    b->d_queue = d.take<int>((int64_t)4 * n);
     for (auto& ev : b->ev) {
         if (hipEventCreate(&ev) != hipSuccess) {
             cf_batch_destroy(b);
@@ -895,11 +908,11 @@ struct NoAllocLayout {
     unsigned* xsync;
     int* queue;
     float *FC, *FM, *FM1, *FM2, *FCX, *F5, *F5o, *logz, *FCo, *FMo, *FM1o, *FM2o, *POST, *SC;
-    int16_t* TB;
     uint8_t* mea_out;
     size_t bytes;
 };
-NoAllocLayout noalloc_layout(char* base, int n, int64_t C, int64_t T, int64_t X, int64_t chars, bool mea) {
+NoAllocLayout noalloc_layout(char* base, int n, int64_t C, int64_t T, int64_t X, int64_t M, int64_t chars,
+                            bool mea) {
     NoAllocLayout l;
     Carver k{base};
     l.seqs = k.take<CfSeq>(n);
@@ -910,7 +923,7 @@ NoAllocLayout noalloc_layout(char* base, int n, int64_t C, int64_t T, int64_t X,
     l.gin = k.take<float>(kMaxGamma + n);
     l.status = k.take<int>(4);
     l.xsync = k.take<unsigned>((int64_t)16 * n);
-    l.queue = k.take<int>((int64_t)2 * n);
+    l.queue = k.take<int>((int64_t)4 * n);
     l.FC = k.take<float>(T);
     l.FM = k.take<float>(T);
     l.FM1 = k.take<float>(T);
@@ -920,7 +933,6 @@ NoAllocLayout noalloc_layout(char* base, int n, int64_t C, int64_t T, int64_t X,
     l.F5o = k.take<float>(C);
     l.logz = k.take<float>(n);
     l.FCo = l.FMo = l.FM1o = l.FM2o = l.POST = l.SC = nullptr;
-    l.TB = nullptr;
     l.mea_out = nullptr;
     if (mea) {
         l.FCo = k.take<float>(T);
@@ -928,8 +940,7 @@ NoAllocLayout noalloc_layout(char* base, int n, int64_t C, int64_t T, int64_t X,
         l.FM1o = k.take<float>(T);
         l.FM2o = k.take<float>(T);
         l.POST = k.take<float>(T);
-        l.SC = k.take<float>(T);
-        l.TB = k.take<int16_t>(T);
+        l.SC = k.take<float>(M);
         l.mea_out = k.take<uint8_t>(chars);
     }
     l.bytes = align_up(k.off, 256);
@@ -969,7 +980,8 @@ int cf_create(int device, cf_ctx** out) {
                          (const void*)cf_outside_kernel<false, true, false>,
                          (const void*)cf_outside_kernel<false, false, true>, (const void*)cf_post_kernel,
                          (const void*)cf_exterior_kernel,
-                         (const void*)cf_mea_kernel, (const void*)cf_viterbi_kernel<false>,
+                         (const void*)cf_mea_kernel<true>, (const void*)cf_mea_kernel<false>,
+                         (const void*)cf_viterbi_kernel<false>,
                          (const void*)cf_viterbi_kernel<true>};
     for (const void* f : dyn) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
     (void)hipGetLastError();
@@ -1083,7 +1095,7 @@ int64_t cf_fold_bytes(int L, unsigned flags, int n_gamma, int n_samples) {
     int64_t bytes = (L + 2) * 2 + sizeof(CfSeq) + 64;  // codes, masks, descriptor
     bytes += (4 * T + X + 2 * (L + 2) + 1) * 4;        // inside tables, FCX, F5, F5o, logZ
     if (flags & (CF_BPP | CF_MEA | CF_GRAD)) bytes += 5 * T * 4;
-    if (flags & CF_MEA) bytes += (int64_t)std::max(1, n_gamma) * (T * 6 + L);
+    if (flags & CF_MEA) bytes += (int64_t)std::max(1, n_gamma) * (cf_mea_cells(L) * 4 + L);
     if (flags & CF_MFE) bytes += 3 * T * 8 + 8 + L;
     if (flags & CF_SAMPLE) bytes += (int64_t)std::max(1, n_samples) * L;
     return bytes;
@@ -1281,24 +1293,27 @@ int cf_fold_noalloc(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, 
         b.Lmax = std::max(b.Lmax, (int)l);
     }
     std::vector<CfSeq> sd(n);
-    int64_t C = 0, T = 0, X = 0;
+    int64_t C = 0, T = 0, X = 0, M = 0;
     for (int i = 0; i < n; ++i) {
         sd[i].L = b.L[i];
         sd[i].coff = (int32_t)C;
         sd[i].toff = T;
         sd[i].xoff = X;
+        sd[i].moff = M;
         C += b.L[i] + 2;
         T += cf_tri_size(b.L[i]);
         X += std::max<int64_t>(cf_x_size(b.L[i]), 1);
+        M += cf_mea_cells(b.L[i]);
     }
-    const NoAllocLayout need = noalloc_layout(nullptr, n, C, T, X, b.total_chars, mea);
+    const NoAllocLayout need = noalloc_layout(nullptr, n, C, T, X, M, b.total_chars, mea);
     if (need.bytes > kScratchBytes)
         return fail(c, CF_EINVAL, "no-alloc fold: batch needs " + std::to_string(need.bytes) +
                                       " bytes of scratch, more than its " + std::to_string(kScratchBytes));
-    const NoAllocLayout l = noalloc_layout((char*)c->scratch, n, C, T, X, b.total_chars, mea);
+    const NoAllocLayout l = noalloc_layout((char*)c->scratch, n, C, T, X, M, b.total_chars, mea);
     b.total_C = C;
     b.total_T = T;
     b.total_X = X;
+    b.total_M = M;
     // one bucket, longest first; a wide team per fold for small calls
     b.order.resize(n);
     for (int i = 0; i < n; ++i) b.order[i] = i;
@@ -1341,7 +1356,7 @@ int cf_fold_noalloc(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, 
     b.FC = l.FC; b.FM = l.FM; b.FM1 = l.FM1; b.FM2 = l.FM2; b.FCX = l.FCX;
     b.F5 = l.F5; b.F5o = l.F5o; b.logz = l.logz;
     b.FCo = l.FCo; b.FMo = l.FMo; b.FM1o = l.FM1o; b.FM2o = l.FM2o; b.POST = l.POST;
-    b.SC = l.SC; b.TB = l.TB; b.mea_out = l.mea_out;
+    b.SC = l.SC; b.mea_out = l.mea_out;
     b.n_gamma = mea ? 1 : 0;
     const unsigned need_flags = mea ? (CF_LOGZ | CF_BPP | CF_MEA) : CF_LOGZ;
     int rc = enqueue_bucket(&b, b.buckets[0], need_flags, st);

@@ -90,6 +90,7 @@ struct CfSeq {
     int32_t coff;   // offset of s[0] in codes / masks (L+2 entries), also F5/F5o/up arrays
     int64_t toff;   // offset of cell (0,0) in every triangular table
     int64_t xoff;   // offset of the exterior FC copy
+    int64_t moff;   // offset of cell (0,0)'s tile in one gamma's tiled MEA score table
 };
 
 // ---------------------------------------------------------------- LSE

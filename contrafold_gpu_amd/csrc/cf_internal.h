@@ -99,7 +99,7 @@ struct cf_batch {
         int split = 1;  // workgroups per fold of the inside / outside / posterior launches
     };
     std::vector<Bucket> buckets;
-    int64_t total_C = 0, total_T = 0, total_X = 0, total_chars = 0;
+    int64_t total_C = 0, total_T = 0, total_X = 0, total_M = 0, total_chars = 0;
     // inputs + inside
     Region r_in, r_inside;
     CfSeq* d_seqs = nullptr;
@@ -113,7 +113,7 @@ struct cf_batch {
     int* d_status = nullptr;    // per-run kernel status bits (1: sampler stack overflow, 2: split team not
                                 // co-resident), zeroed by each run
     unsigned* d_xsync = nullptr;  // split-team barrier counters: 16 words per sequence (by launch order)
-    int* d_queue = nullptr;       // fold queues of persistent packed launches: 2 per sequence (by launch order)
+    int* d_queue = nullptr;       // fold queues of persistent packed launches: 4 per sequence (by launch order)
     float *FC = nullptr, *FM = nullptr, *FM1 = nullptr, *FM2 = nullptr, *FCX = nullptr, *F5 = nullptr,
           *F5o = nullptr, *logz = nullptr;
     // outside
@@ -122,8 +122,7 @@ struct cf_batch {
     // MEA
     Region r_mea;
     int mea_cap = 0, n_gamma = 0;
-    float* SC = nullptr;
-    int16_t* TB = nullptr;
+    float* SC = nullptr;  // tiled MEA score tables, total_M floats per gamma
     uint8_t* mea_out = nullptr;
     // Viterbi
     Region r_vit;

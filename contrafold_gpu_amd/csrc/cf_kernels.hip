@@ -1472,133 +1472,289 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
 }
 
 // ======================================================================= MEA
-// Maximum-expected-accuracy decoding (cpu.py:823-895).  Max-plus values are
-// exact, so the strict '>' scan over the candidate list equals a parallel
-// argmax with lowest-index ties; here each lane still scans its own cell in
-// order, which is both exact and simple.  Output: dot-bracket bytes.
+// Maximum-expected-accuracy decoding (cpu.py:823-895).
+//
+// The score table S (cpu.py:843-871) is a max-plus recurrence: each candidate
+// is one FP32 addition of two table values, and a maximum does not depend on
+// the order its operands are compared in.  So S is computed here in an order
+// chosen for data reuse, bit-identical to the reference's.  The reference's
+// strict '>' scan only decides the traceback: it keeps the first candidate,
+// in its order (i+1 unpaired, j unpaired, pair (i+1, j), bifurcations k
+// ascending), that reaches the cell's maximum; the traceback finds that
+// candidate again by replaying the order against S.
+//
+// Layout: S of one fold is stored in 16 x 16 tiles, tile (I, J) holding rows
+// 16I.. and columns 16J.. row-major, the tiles of block diagonal D = J - I
+// stored together, D ascending (cf_mea_tile).  A cell's bifurcation operands
+// S[i][k], S[k][j] lie in tiles (I, K) and (K, J):
+//   far  (I < K < J): both tiles lie on block diagonals below D.  Phase 1
+//        stages the pair in LDS and folds a max-plus tile product into the
+//        cell (each loaded value serves 16 cells);
+//   near (K = I or K = J): one operand lies in the cell's own tile.  Phase 2
+//        walks the tile's cells by w = c - r (in-tile column minus row):
+//        the near operands of a cell lie on smaller w, and so do the
+//        unpaired / pair candidates' cells.  Every cell of one step has
+//        15 + w near terms (w - 1 in a diagonal tile), so lanes stay
+//        converged.
+// Teams: PACK -- one wave per (fold, gamma), several per workgroup, taking
+// items from a queue; otherwise the workgroup is one team (phase-1 tiles
+// spread over its waves, phase-2 cells over its threads).
+#define CF_MT 16
+__host__ __device__ inline int cf_mea_nb(int L) { return (L + CF_MT) / CF_MT; }  // ceil((L + 1) / 16)
+__host__ __device__ inline int64_t cf_mea_cells(int L) {
+    const int64_t nb = cf_mea_nb(L);
+    return nb * (nb + 1) / 2 * (CF_MT * CF_MT);
+}
+// tile (I, I + D) of a fold with nb tile rows
+__device__ __forceinline__ int cf_mea_tile(int nb, int I, int D) { return D * nb - ((D * (D - 1)) >> 1) + I; }
+// element offset of cell (i, j), i <= j
+__device__ __forceinline__ int cf_mea_at(int nb, int i, int j) {
+    const int I = i >> 4, J = j >> 4;
+    return (cf_mea_tile(nb, I, J - I) << 8) | ((i & 15) << 4) | (j & 15);
+}
+// LDS of one team: per-wave tile staging (A transposed, B row-major), then
+// up[], the traceback stack, codes and masks.
+struct CfMeaLds {
+    int o_up, o_st, o_s8, o_fo8, team;
+};
+__host__ __device__ inline CfMeaLds cf_mea_plan(int Lmax, int nw) {
+    CfMeaLds p;
+    int o = nw * 2 * CF_MT * CF_MT * 4;
+    p.o_up = o; o += cf_align((Lmax + 2) * 4, 16);
+    p.o_st = o; o += cf_align(2 * (Lmax + 2) * 4, 16);
+    p.o_s8 = o; o += cf_align(Lmax + 3, 16);
+    p.o_fo8 = o; o += cf_align(Lmax + 3, 16);
+    p.team = o;
+    return p;
+}
+__device__ __forceinline__ void cf_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+__device__ __forceinline__ int cf_dbase32(int L, int d) { return d * (L + 1) - ((d * (d - 1)) >> 1); }
+
+template <bool PACK>
 __global__ void __launch_bounds__(512) cf_mea_kernel(const CfSeq* __restrict__ seqs, const int* __restrict__ order,
-                                                     int nlaunch, const uint8_t* __restrict__ codes,
-                                                     const uint8_t* __restrict__ masks, const float* __restrict__ POSTF,
-                                                     const float* __restrict__ gammas, float* __restrict__ SC,
-                                                     int16_t* __restrict__ TB, int64_t tab_stride,
+                                                     int nl, int n_gamma, const uint8_t* __restrict__ codes,
+                                                     const uint8_t* __restrict__ masks,
+                                                     const float* __restrict__ POSTF, const float* __restrict__ gammas,
+                                                     float* __restrict__ SC, int64_t tab_stride,
                                                      uint8_t* __restrict__ out, const int64_t* __restrict__ out_off,
-                                                     int64_t out_stride, int Lmax) {
+                                                     int64_t out_stride, int Lmax, int* __restrict__ queue) {
     extern __shared__ __align__(16) unsigned char smem[];
-    float* up = (float*)smem;                          // L+2
-    int8_t* s8 = (int8_t*)(up + (Lmax + 2));           // L+2
-    int8_t* fo8 = s8 + (Lmax + 2);                     // L+2
-    int* qs = (int*)(((uintptr_t)(fo8 + Lmax + 2) + 15) & ~(uintptr_t)15);  // traceback stack 2*(L+2)
-    const int li = blockIdx.x % nlaunch;
-    const int g = blockIdx.x / nlaunch;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nw = PACK ? 1 : (int)(blockDim.x / WAVE);
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int team = PACK ? wid : 0;
+    const int wv = PACK ? 0 : wid;  // wave within the team
+    const int tid = PACK ? lane : (int)threadIdx.x;
+    const int NT = PACK ? WAVE : (int)blockDim.x;
+    const int G = PACK ? (int)(blockDim.x / WAVE) : 1;
+    const CfMeaLds lp = cf_mea_plan(Lmax, nw);
+    unsigned char* tarea = smem + team * lp.team;
+    float* As = (float*)tarea + wv * 2 * CF_MT * CF_MT;  // As[k * 16 + r] = A[r][k]
+    float* Bs = As + CF_MT * CF_MT;                      // Bs[k * 16 + c] = B[k][c]
+    float* up = (float*)(tarea + lp.o_up);
+    uint32_t* st = (uint32_t*)(tarea + lp.o_st);
+    int8_t* s8 = (int8_t*)(tarea + lp.o_s8);
+    int8_t* fo8 = (int8_t*)(tarea + lp.o_fo8);
+    const int nitem = nl * n_gamma;
+    const bool persistent = PACK && queue != nullptr;
+    for (int item = persistent ? cf_next_fold(queue) : (PACK ? (int)blockIdx.x * G + team : (int)blockIdx.x);
+         item < nitem; item = persistent ? cf_next_fold(queue) : nitem) {
+    if (persistent) cf_team_sync<PACK>();  // the team's LDS area is reused item after item
+    const int li = item / n_gamma, g = item - li * n_gamma;
     const int bseq = order[li];
     const CfSeq sd = seqs[bseq];
     const int L = sd.L;
-    const int tid = threadIdx.x, NT = blockDim.x;
-    const float* __restrict__ post = POSTF + sd.toff;
-    float* __restrict__ sc = SC + g * tab_stride + sd.toff;
-    int16_t* __restrict__ tb = TB + g * tab_stride + sd.toff;
+    const int nb = cf_mea_nb(L);
+    const float* __restrict__ post = POSTF + sd.toff;  // POST(a, b+1) at triangle cell (a, b)
+    float* __restrict__ S = SC + g * tab_stride + sd.moff;
     for (int q = tid; q < L + 2; q += NT) {
         s8[q] = (int8_t)codes[sd.coff + q];
         fo8[q] = (masks != nullptr && masks[sd.coff + q] != 0) ? 1 : 0;
     }
-    const float gam = gammas[g];
-    const float two_g = 2.0f * gam;
-    // up[i] = (1 - column sum - row sum) / (2 gamma); POST(x, y) = POSTF[tri(x, y-1)]
-    for (int i = 1 + tid; i <= L; i += NT) {
+    // up[i] = (1 - column sum - row sum) / (2 gamma), subtracted in the
+    // reference's order (cpu.py:831-838): POST(j, i) for j = 1 .. i-1, then
+    // POST(i, j) for j = i+1 .. L.  The threads of a block of consecutive i
+    // step through one triangle diagonal together, so each load is coalesced:
+    // the column term j = i - d lies on diagonal d - 1 (d descending), the row
+    // term j = i + 1 + m on diagonal m (m ascending).
+    const float two_g = 2.0f * gammas[g];
+    for (int i0 = 0; i0 < L; i0 += NT) {
+        const int i = 1 + i0 + tid;
+        const bool on = i <= L;
         float u = 1.0f;
-        for (int j = 1; j < i; ++j) u -= post[cf_tri(L, j, i - 1)];
-        for (int j = i + 1; j <= L; ++j) u -= post[cf_tri(L, i, j - 1)];
-        up[i] = u / two_g;
+        const int dtop = min(i0 + NT, L) - 1;
+        for (int d0 = dtop; d0 >= 1; d0 -= 8) {
+            float x[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int d = d0 - t;
+                const bool act = on && d >= 1 && d <= i - 1;
+                const float v = post[act ? cf_dbase32(L, d - 1) + i - d : 0];
+                x[t] = act ? v : 0.0f;
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) u -= x[t];  // an inactive slot subtracts +0: u unchanged
+        }
+        const int mtop = L - 2 - i0;  // L - 1 - i at the block's first i
+        for (int m0 = 0; m0 <= mtop; m0 += 8) {
+            float x[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int m = m0 + t;
+                const bool act = on && m <= L - 1 - i;
+                const float v = post[act ? cf_dbase32(L, m) + i : 0];
+                x[t] = act ? v : 0.0f;
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) u -= x[t];
+        }
+        if (on) up[i] = u / two_g;
     }
-    __syncthreads();
-    for (int s = 0; s <= L; ++s) {
-        const int64_t db = cf_dbase(L, s);
-        for (int i = tid; i <= L - s; i += NT) {
-            const int j = i + s;
-            float best = -1.0f;
-            int t = -1;
-            if (s == 0) {
-                best = 0.0f;
-                t = 0;
-            } else {
-                float v = up[i + 1] + sc[cf_dbase(L, s - 1) + i + 1];
-                if (v > best) { best = v; t = 1; }
-                v = up[j] + sc[cf_dbase(L, s - 1) + i];
-                if (v > best) { best = v; t = 2; }
-                if (s >= 2) {
-                    if (cf_canon(s8[i + 1], s8[j]) && !fo8[i + 1] && !fo8[j]) {
-                        v = post[cf_dbase(L, s - 2) + i + 1] + sc[cf_dbase(L, s - 2) + i + 1];
-                        if (v > best) { best = v; t = 3; }
+    cf_team_sync<PACK>();
+
+    for (int D = 0; D < nb; ++D) {
+        const int ntile = nb - D;
+        // ---- phase 1: far bifurcations of the tiles (I, I+D), D >= 2
+        if (D >= 2) {
+            const int r = lane >> 2, c0 = (lane & 3) << 2;
+            for (int I = wv; I < ntile; I += nw) {
+                const int J = I + D;
+                float4 acc = make_float4(CF_NEG, CF_NEG, CF_NEG, CF_NEG);
+                // tile (I, K) of block diagonal K - I, tile (K, J) of J - K
+                auto ldA = [&](int K) { return *(const float4*)(S + (cf_mea_tile(nb, I, K - I) << 8) + 4 * lane); };
+                auto ldB = [&](int K) { return *(const float4*)(S + (cf_mea_tile(nb, K, J - K) << 8) + 4 * lane); };
+                float4 an = ldA(I + 1), bn = ldB(I + 1);
+                for (int K = I + 1; K < J; ++K) {
+                    cf_wave_sync();  // the previous product has read the staged tiles
+                    As[(c0 + 0) * CF_MT + r] = an.x;
+                    As[(c0 + 1) * CF_MT + r] = an.y;
+                    As[(c0 + 2) * CF_MT + r] = an.z;
+                    As[(c0 + 3) * CF_MT + r] = an.w;
+                    *(float4*)(Bs + 4 * lane) = bn;
+                    cf_wave_sync();
+                    if (K + 1 < J) {
+                        an = ldA(K + 1);
+                        bn = ldB(K + 1);
                     }
-                    // splits d = 1 .. s-1 compared in k order; the loads of the
-                    // next 8 splits are in flight while the current 8 compare,
-                    // and past the last split the loads repeat its address and
-                    // the compare is skipped
-                    int ia = (int)cf_dbase(L, 1) + i;
-                    int ib = (int)cf_dbase(L, s - 1) + i + 1;
-                    int dn = 1;
-                    auto load = [&](float (&x)[8], float (&y)[8]) {
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            x[u] = cf_ld(sc, ia);
-                            y[u] = cf_ld(sc, ib);
-                            const bool adv = dn < s - 1;
-                            ia += adv ? L + 1 - dn : 0;
-                            ib -= adv ? L - (s - dn - 1) : 0;
-                            ++dn;
-                        }
-                    };
-                    auto cmp = [&](const float (&x)[8], const float (&y)[8], int d0) {
+                    for (int k = 0; k < CF_MT; ++k) {
+                        const float a = As[k * CF_MT + r];
+                        const float4 bv = *(const float4*)(Bs + k * CF_MT + c0);
+                        acc.x = fmaxf(acc.x, a + bv.x);
+                        acc.y = fmaxf(acc.y, a + bv.y);
+                        acc.z = fmaxf(acc.z, a + bv.z);
+                        acc.w = fmaxf(acc.w, a + bv.w);
+                    }
+                }
+                *(float4*)(S + (cf_mea_tile(nb, I, D) << 8) + 4 * lane) = acc;
+            }
+            cf_team_sync<PACK>();
+        }
+        // ---- phase 2: the in-tile wavefront
+        for (int w = (D == 0 ? 0 : 1 - CF_MT); w < CF_MT; ++w) {
+            const int aw = w < 0 ? -w : w;
+            const int nr = CF_MT - aw, rlo = w < 0 ? -w : 0;
+            const int ncell = ntile * nr;
+            const int nt = D == 0 ? w - 1 : CF_MT - 1 + w;  // near terms of every cell of this step
+            for (int q = tid; q < ncell; q += NT) {
+                const int I = q / nr;
+                const int r = rlo + (q - I * nr), c = r + w;
+                const int J = I + D;
+                const int i = CF_MT * I + r, j = CF_MT * J + c;
+                if (j > L) continue;
+                const int self = (cf_mea_tile(nb, I, D) << 8) | (r << 4) | c;
+                float best = 0.0f;
+                if (j > i) {
+                    best = -1.0f;
+                    if (D >= 2) best = fmaxf(best, S[self]);
+                    best = fmaxf(best, up[i + 1] + S[cf_mea_at(nb, i + 1, j)]);
+                    best = fmaxf(best, up[j] + S[cf_mea_at(nb, i, j - 1)]);
+                    if (j - i >= 2 && cf_canon(s8[i + 1], s8[j]) && !fo8[i + 1] && !fo8[j])
+                        best = fmaxf(best, post[cf_dbase32(L, j - i - 2) + i + 1] + S[cf_mea_at(nb, i + 1, j - 1)]);
+                    // near bifurcations: k in block I below row i (S[i][k] in tile (I, I),
+                    // S[k][j] in this tile), then k in block J left of column j (S[i][k]
+                    // in this tile, S[k][j] in tile (J, J)); past the last term the
+                    // index repeats it, which leaves a maximum unchanged
+                    const int n1 = D == 0 ? nt : CF_MT - 1 - r;
+                    const int tII = cf_mea_tile(nb, I, 0) << 8, tIJ = cf_mea_tile(nb, I, D) << 8,
+                              tJJ = cf_mea_tile(nb, J, 0) << 8;
+                    for (int m0 = 0; m0 < nt; m0 += 8) {
+                        float xa[8], xb[8];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            const float w = x[u] + y[u];
-                            if (d0 + u < s && w > best) { best = w; t = i + d0 + u + 4; }
+                        for (int t = 0; t < 8; ++t) {
+                            const int m = min(m0 + t, nt - 1);
+                            const bool lo = m < n1;
+                            const int kk = lo ? r + 1 + m : m - n1;
+                            xa[t] = S[lo ? tII + (r << 4) + kk : tIJ + (r << 4) + kk];
+                            xb[t] = S[lo ? tIJ + (kk << 4) + c : tJJ + (kk << 4) + c];
                         }
-                    };
-                    const int nb = (s - 1 + 7) >> 3;
-                    float xa[8], ya[8], xb[8], yb[8];
-                    load(xa, ya);
-                    for (int k = 0; k < nb; k += 2) {
-                        load(xb, yb);
-                        cmp(xa, ya, 1 + 8 * k);
-                        if (k + 1 >= nb) break;
-                        load(xa, ya);
-                        cmp(xb, yb, 1 + 8 * (k + 1));
+#pragma unroll
+                        for (int t = 0; t < 8; ++t) best = fmaxf(best, xa[t] + xb[t]);
+                    }
+                }
+                S[self] = best;
+            }
+            cf_team_sync<PACK>();
+        }
+    }
+
+    // ---- traceback (cpu.py:873-895): replay of the reference's candidate order
+    uint8_t* __restrict__ o = out + g * out_stride + out_off[bseq];
+    for (int q = tid; q < L; q += NT) o[q] = '.';
+    cf_team_sync<PACK>();
+    if (wv == 0) {
+        auto at = [&](int a, int b) { return S[cf_mea_at(nb, a, b)]; };
+        if (lane == 0) st[0] = (uint32_t)L << 16;
+        int sp = 1;
+        cf_wave_sync();
+        while (sp > 0) {
+            --sp;
+            const uint32_t e0 = st[sp];
+            const int i = (int)(e0 & 0xffffu), j = (int)(e0 >> 16);
+            if (j == i) continue;
+            const float best = at(i, j);
+            if (!(best > -1.0f)) continue;  // the reference's tb stays -1
+            int push = 0;
+            uint32_t p0 = 0u, p1 = 0u;
+            if (up[i + 1] + at(i + 1, j) == best) {
+                push = 1; p0 = (uint32_t)(i + 1) | ((uint32_t)j << 16);
+            } else if (up[j] + at(i, j - 1) == best) {
+                push = 1; p0 = (uint32_t)i | ((uint32_t)(j - 1) << 16);
+            } else if (j - i >= 2 && cf_canon(s8[i + 1], s8[j]) && !fo8[i + 1] && !fo8[j] &&
+                       post[cf_dbase32(L, j - i - 2) + i + 1] + at(i + 1, j - 1) == best) {
+                if (lane == 0) {
+                    o[i] = '(';
+                    o[j - 1] = ')';
+                }
+                push = 1; p0 = (uint32_t)(i + 1) | ((uint32_t)(j - 1) << 16);
+            } else {
+                // the first bifurcation k (ascending) that reaches the maximum
+                for (int k0 = i + 1; k0 < j; k0 += WAVE) {
+                    const int k = k0 + lane;
+                    const bool hit = k < j && at(i, k) + at(k, j) == best;
+                    const uint64_t m = __ballot(hit);
+                    if (m != 0ull) {
+                        const int ks = k0 + (int)__builtin_ctzll(m);
+                        push = 2;
+                        p0 = (uint32_t)i | ((uint32_t)ks << 16);
+                        p1 = (uint32_t)ks | ((uint32_t)j << 16);
+                        break;
                     }
                 }
             }
-            sc[db + i] = best;
-            tb[db + i] = (int16_t)t;
-        }
-        __syncthreads();
-    }
-    // traceback (cpu.py:873-895), dot-bracket output
-    uint8_t* __restrict__ o = out + g * out_stride + out_off[bseq];
-    for (int q = tid; q < L; q += NT) o[q] = '.';
-    __syncthreads();
-    if (tid == 0) {
-        int sp = 0;
-        qs[0] = 0;
-        qs[1] = L;
-        sp = 1;
-        while (sp > 0) {
-            --sp;
-            const int i = qs[2 * sp], j = qs[2 * sp + 1];
-            const int t = tb[cf_tri(L, i, j)];
-            if (t <= 0) continue;
-            if (t == 1) { qs[2 * sp] = i + 1; qs[2 * sp + 1] = j; ++sp; }
-            else if (t == 2) { qs[2 * sp] = i; qs[2 * sp + 1] = j - 1; ++sp; }
-            else if (t == 3) {
-                o[i] = '(';
-                o[j - 1] = ')';
-                qs[2 * sp] = i + 1; qs[2 * sp + 1] = j - 1; ++sp;
-            } else {
-                const int k = t - 4;
-                qs[2 * sp] = i; qs[2 * sp + 1] = k; ++sp;
-                qs[2 * sp] = k; qs[2 * sp + 1] = j; ++sp;
+            if (lane == 0) {
+                if (push >= 1) st[sp] = p0;
+                if (push == 2) st[sp + 1] = p1;
             }
+            sp += push;
+            cf_wave_sync();
         }
     }
+    }  // items of this team
 }
 
 // =================================================================== FP64
