@@ -1484,8 +1484,10 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
 // candidate again by replaying the order against S.
 //
 // Layout: S of one fold is stored in 16 x 16 tiles, tile (I, J) holding rows
-// 16I.. and columns 16J.. row-major, the tiles of block diagonal D = J - I
-// stored together, D ascending (cf_mea_tile).  A cell's bifurcation operands
+// 16I.. and columns 16J.., the tiles of block diagonal D = J - I stored
+// together, D ascending (cf_mea_tile).  Inside a tile the cells are stored by
+// in-tile diagonals d = c - r ascending (cf_mea_tix), so the cells a step
+// touches -- one in-tile diagonal per operand -- are contiguous across lanes.  A cell's bifurcation operands
 // S[i][k], S[k][j] lie in tiles (I, K) and (K, J):
 //   far  (I < K < J): both tiles lie on block diagonals below D.  Phase 1
 //        stages the pair in LDS and folds a max-plus tile product into the
@@ -1507,10 +1509,19 @@ __host__ __device__ inline int64_t cf_mea_cells(int L) {
 }
 // tile (I, I + D) of a fold with nb tile rows
 __device__ __forceinline__ int cf_mea_tile(int nb, int I, int D) { return D * nb - ((D * (D - 1)) >> 1) + I; }
+// offset of in-tile diagonal d (-15 .. 16) within a tile: diagonal d holds
+// 16 - |d| cells, ordered by their smaller coordinate
+__host__ __device__ __forceinline__ int cf_mea_doff(int d) {
+    return d <= 0 ? (((15 + d) * (16 + d)) >> 1) : 120 + 16 * d - ((d * (d - 1)) >> 1);
+}
+__host__ __device__ __forceinline__ int cf_mea_tix(int r, int c) {
+    const int d = c - r;
+    return cf_mea_doff(d) + (d <= 0 ? c : r);
+}
 // element offset of cell (i, j), i <= j
 __device__ __forceinline__ int cf_mea_at(int nb, int i, int j) {
     const int I = i >> 4, J = j >> 4;
-    return (cf_mea_tile(nb, I, J - I) << 8) | ((i & 15) << 4) | (j & 15);
+    return (cf_mea_tile(nb, I, J - I) << 8) + cf_mea_tix(i & 15, j & 15);
 }
 // LDS of one team: per-wave tile staging (A transposed, B row-major), then
 // up[], the traceback stack, codes and masks.
@@ -1559,6 +1570,17 @@ __global__ void __launch_bounds__(512) cf_mea_kernel(const CfSeq* __restrict__ s
     uint32_t* st = (uint32_t*)(tarea + lp.o_st);
     int8_t* s8 = (int8_t*)(tarea + lp.o_s8);
     int8_t* fo8 = (int8_t*)(tarea + lp.o_fo8);
+    // phase 1: the tile elements 4 lane .. 4 lane + 3 a lane loads, as (r, c)
+    int er[4], ec[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        const int e = 4 * lane + x;
+        int d = -15;
+        while (d < 15 && cf_mea_doff(d + 1) <= e) ++d;
+        const int p = e - cf_mea_doff(d);
+        er[x] = d <= 0 ? p - d : p;
+        ec[x] = d <= 0 ? p : p + d;
+    }
     const int nitem = nl * n_gamma;
     const bool persistent = PACK && queue != nullptr;
     for (int item = persistent ? cf_next_fold(queue) : (PACK ? (int)blockIdx.x * G + team : (int)blockIdx.x);
@@ -1618,6 +1640,7 @@ __global__ void __launch_bounds__(512) cf_mea_kernel(const CfSeq* __restrict__ s
 
     for (int D = 0; D < nb; ++D) {
         const int ntile = nb - D;
+        const int t0D = cf_mea_tile(nb, 0, D) << 8;  // tile (I, I + D) starts at t0D + 256 I
         // ---- phase 1: far bifurcations of the tiles (I, I+D), D >= 2
         if (D >= 2) {
             const int r = lane >> 2, c0 = (lane & 3) << 2;
@@ -1630,11 +1653,14 @@ __global__ void __launch_bounds__(512) cf_mea_kernel(const CfSeq* __restrict__ s
                 float4 an = ldA(I + 1), bn = ldB(I + 1);
                 for (int K = I + 1; K < J; ++K) {
                     cf_wave_sync();  // the previous product has read the staged tiles
-                    As[(c0 + 0) * CF_MT + r] = an.x;
-                    As[(c0 + 1) * CF_MT + r] = an.y;
-                    As[(c0 + 2) * CF_MT + r] = an.z;
-                    As[(c0 + 3) * CF_MT + r] = an.w;
-                    *(float4*)(Bs + 4 * lane) = bn;
+                    As[ec[0] * CF_MT + er[0]] = an.x;
+                    As[ec[1] * CF_MT + er[1]] = an.y;
+                    As[ec[2] * CF_MT + er[2]] = an.z;
+                    As[ec[3] * CF_MT + er[3]] = an.w;
+                    Bs[er[0] * CF_MT + ec[0]] = bn.x;
+                    Bs[er[1] * CF_MT + ec[1]] = bn.y;
+                    Bs[er[2] * CF_MT + ec[2]] = bn.z;
+                    Bs[er[3] * CF_MT + ec[3]] = bn.w;
                     cf_wave_sync();
                     if (K + 1 < J) {
                         an = ldA(K + 1);
@@ -1650,7 +1676,11 @@ __global__ void __launch_bounds__(512) cf_mea_kernel(const CfSeq* __restrict__ s
                         acc.w = fmaxf(acc.w, a + bv.w);
                     }
                 }
-                *(float4*)(S + (cf_mea_tile(nb, I, D) << 8) + 4 * lane) = acc;
+                float* __restrict__ dst = S + (cf_mea_tile(nb, I, D) << 8);
+                dst[cf_mea_tix(r, c0 + 0)] = acc.x;
+                dst[cf_mea_tix(r, c0 + 1)] = acc.y;
+                dst[cf_mea_tix(r, c0 + 2)] = acc.z;
+                dst[cf_mea_tix(r, c0 + 3)] = acc.w;
             }
             cf_team_sync<PACK>();
         }
@@ -1666,7 +1696,7 @@ __global__ void __launch_bounds__(512) cf_mea_kernel(const CfSeq* __restrict__ s
                 const int J = I + D;
                 const int i = CF_MT * I + r, j = CF_MT * J + c;
                 if (j > L) continue;
-                const int self = (cf_mea_tile(nb, I, D) << 8) | (r << 4) | c;
+                const int self = (cf_mea_tile(nb, I, D) << 8) + cf_mea_doff(w) + (w <= 0 ? c : r);
                 float best = 0.0f;
                 if (j > i) {
                     best = -1.0f;
@@ -1678,22 +1708,31 @@ __global__ void __launch_bounds__(512) cf_mea_kernel(const CfSeq* __restrict__ s
                     // near bifurcations: k in block I below row i (S[i][k] in tile (I, I),
                     // S[k][j] in this tile), then k in block J left of column j (S[i][k]
                     // in this tile, S[k][j] in tile (J, J)); past the last term the
-                    // index repeats it, which leaves a maximum unchanged
-                    const int n1 = D == 0 ? nt : CF_MT - 1 - r;
-                    const int tII = cf_mea_tile(nb, I, 0) << 8, tIJ = cf_mea_tile(nb, I, D) << 8,
-                              tJJ = cf_mea_tile(nb, J, 0) << 8;
-                    for (int m0 = 0; m0 < nt; m0 += 8) {
-                        float xa[8], xb[8];
+                    // index repeats it, which leaves a maximum unchanged.  Slot m of
+                    // either kind reads one in-tile diagonal per operand at position
+                    // r + a uniform offset, so the lanes of a tile read contiguous cells.
+                    // address = lb (per lane: 256 I + r) + a wave-uniform offset per slot
+                    const int lb = (I << 8) + r;
+                    const int n1 = D == 0 ? CF_MT : CF_MT - 1 - r;  // slots m < n1 are lo
+                    for (int mb = 0; mb < nt; mb += 16) {
+                        float xa[16], xb[16];
 #pragma unroll
-                        for (int t = 0; t < 8; ++t) {
-                            const int m = min(m0 + t, nt - 1);
+                        for (int t = 0; t < 16; ++t) {
+                            const int m = min(mb + t, nt - 1);
+                            // lo: S[i][i+1+m] (diagonal m+1 of tile (I, I)), S[i+1+m][j]
+                            // (diagonal w-1-m of this tile)
+                            const int alo = cf_mea_doff(m + 1);
+                            const int blo = t0D + cf_mea_doff(w - 1 - m) + min(1 + m, w);
+                            // hi, k = 16J + m - 15 + r: S[i][k] (diagonal m-15 of this tile),
+                            // S[k][j] (diagonal w-m+15 of tile (J, J))
+                            const int ahi = t0D + cf_mea_doff(m - 15) + min(m - 15, 0);
+                            const int bhi = (D << 8) + cf_mea_doff(w - m + 15) + m - 15;
                             const bool lo = m < n1;
-                            const int kk = lo ? r + 1 + m : m - n1;
-                            xa[t] = S[lo ? tII + (r << 4) + kk : tIJ + (r << 4) + kk];
-                            xb[t] = S[lo ? tIJ + (kk << 4) + c : tJJ + (kk << 4) + c];
+                            xa[t] = cf_ld(S, lb + (lo ? alo : ahi));
+                            xb[t] = cf_ld(S, lb + (lo ? blo : bhi));
                         }
 #pragma unroll
-                        for (int t = 0; t < 8; ++t) best = fmaxf(best, xa[t] + xb[t]);
+                        for (int t = 0; t < 16; ++t) best = fmaxf(best, xa[t] + xb[t]);
                     }
                 }
                 S[self] = best;
@@ -1716,16 +1755,20 @@ __global__ void __launch_bounds__(512) cf_mea_kernel(const CfSeq* __restrict__ s
             const uint32_t e0 = st[sp];
             const int i = (int)(e0 & 0xffffu), j = (int)(e0 >> 16);
             if (j == i) continue;
+            // the cell and its unpaired / pair candidates' operands, loaded together
+            const bool two = j - i >= 2;
             const float best = at(i, j);
+            const float x1 = at(i + 1, j), x2 = at(i, j - 1);
+            const float x3 = at(i + 1, two ? j - 1 : j);
+            const float p3 = post[two ? cf_dbase32(L, j - i - 2) + i + 1 : 0];
             if (!(best > -1.0f)) continue;  // the reference's tb stays -1
             int push = 0;
             uint32_t p0 = 0u, p1 = 0u;
-            if (up[i + 1] + at(i + 1, j) == best) {
+            if (up[i + 1] + x1 == best) {
                 push = 1; p0 = (uint32_t)(i + 1) | ((uint32_t)j << 16);
-            } else if (up[j] + at(i, j - 1) == best) {
+            } else if (up[j] + x2 == best) {
                 push = 1; p0 = (uint32_t)i | ((uint32_t)(j - 1) << 16);
-            } else if (j - i >= 2 && cf_canon(s8[i + 1], s8[j]) && !fo8[i + 1] && !fo8[j] &&
-                       post[cf_dbase32(L, j - i - 2) + i + 1] + at(i + 1, j - 1) == best) {
+            } else if (two && cf_canon(s8[i + 1], s8[j]) && !fo8[i + 1] && !fo8[j] && p3 + x3 == best) {
                 if (lane == 0) {
                     o[i] = '(';
                     o[j - 1] = ')';
