@@ -1956,28 +1956,38 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
         const int n0 = L + 1 - s;
         const int n1 = (s + 1 <= L) ? L - s : 0;
         // ---- A
-        for (int i = tid; i < n0; i += NT) {
+        // Max-plus (LSE = false): a maximum does not depend on the order of
+        // its terms, so when a diagonal has fewer rows than threads, a group
+        // of g lanes (a power of two, within one wave) shares a row's split
+        // terms (t = 2 + sub, 2 + sub + g, ...) and the group takes the
+        // maximum of its partial results.  The FP64 log-sum-exp keeps g = 1.
+        int ga = 1;
+        if (!LSE)
+            while (ga < 8 && n0 * ga * 2 <= NT) ga <<= 1;
+        const int suba = tid & (ga - 1);
+        for (int i = tid / ga; i < n0; i += NT / ga) {
             const int j0 = i + s;
             const bool inner0 = (i > 0) && (j0 < L);
             const bool inner1 = (i < n1) && (i > 0) && (j0 + 1 < L);
             double a0 = CF_NEG64, a1 = CF_NEG64;
             if (inner0 && s >= 3) {
                 // t = 2..s-2 for both cells (FM1[i][i+t] shared), t = s-1 for (i, i+s+1)
-                int64_t ia = cf_dbase(L, 2) + i;
-                int64_t ib0 = cf_dbase(L, s - 2) + i + 2;
-                int64_t ib1 = cf_dbase(L, s - 1) + i + 2;
-                int t = 2;
+                int t = 2 + suba;
+                int64_t ia = cf_dbase(L, t) + i;
+                int64_t ib0 = cf_dbase(L, s - t) + i + t;
+                int64_t ib1 = cf_dbase(L, s + 1 - t) + i + t;
                 constexpr int VB = 4;  // FP64 terms per batch (3 x 4 doubles in flight)
-                for (; t + VB - 1 <= s - 2; t += VB) {
+                for (; t + (VB - 1) * ga <= s - 2; t += VB * ga) {
                     double x[VB], y0[VB], y1[VB];
 #pragma unroll
                     for (int u = 0; u < VB; ++u) {
                         x[u] = cf_ld(fm1_t, ia);
                         y0[u] = cf_ld(fm_t, ib0);
                         y1[u] = cf_ld(fm_t, ib1);
-                        ia += L + 1 - (t + u);
-                        ib0 -= L - (s - (t + u) - 1);
-                        ib1 -= L - (s - (t + u));
+                        const int tu = t + u * ga;
+                        ia = cf_dbase(L, tu + ga) + i;
+                        ib0 = cf_dbase(L, s - tu - ga) + i + tu + ga;
+                        ib1 = cf_dbase(L, s + 1 - tu - ga) + i + tu + ga;
                     }
 #pragma unroll
                     for (int u = 0; u < VB; ++u) {
@@ -1985,18 +1995,27 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                         if (x[u] > CF_HALF64 && y1[u] > CF_HALF64) cf_acc<LSE>(a1, x[u] + y1[u]);
                     }
                 }
-                for (; t <= s - 2; ++t) {
+                for (; t <= s - 2; t += ga) {
                     const double x = cf_ld(fm1_t, ia), y0 = cf_ld(fm_t, ib0), y1 = cf_ld(fm_t, ib1);
                     if (x > CF_HALF64 && y0 > CF_HALF64) cf_acc<LSE>(a0, x + y0);
                     if (x > CF_HALF64 && y1 > CF_HALF64) cf_acc<LSE>(a1, x + y1);
-                    ia += L + 1 - t;
-                    ib0 -= L - (s - t - 1);
-                    ib1 -= L - (s - t);
+                    ia = cf_dbase(L, t + ga) + i;
+                    ib0 = cf_dbase(L, s - t - ga) + i + t + ga;
+                    ib1 = cf_dbase(L, s + 1 - t - ga) + i + t + ga;
                 }
-                if (inner1) {
+                // t = s - 1 for (i, i+s+1) alone, by the lane of the group it falls to
+                if (inner1 && t == s - 1) {
                     const double x = cf_ld(fm1_t, ia), y1 = cf_ld(fm_t, ib1);
                     if (x > CF_HALF64 && y1 > CF_HALF64) cf_acc<LSE>(a1, x + y1);
                 }
+            }
+            if (!LSE) {
+                for (int off = 1; off < ga; off <<= 1) {
+                    const double o0 = __shfl_xor(a0, off), o1 = __shfl_xor(a1, off);
+                    a0 = o0 > a0 ? o0 : a0;
+                    a1 = o1 > a1 ? o1 : a1;
+                }
+                if (suba != 0) continue;
             }
 #pragma unroll
             for (int hi = 0; hi < 2; ++hi) {
@@ -2013,9 +2032,16 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
         }
         __syncthreads();
         // ---- B
+        // Max-plus: a group of gb lanes shares a cell's window by rows
+        // (p = i + sub, i + sub + gb, ...) when the diagonal pair has fewer
+        // pairable cells than threads, and takes the maximum of its parts.
         const int nal = cnt[step & 1];
         if (tid == 0) cnt[(step + 1) & 1] = 0;
-        for (int idx = tid; idx < nal; idx += NT) {
+        int gb = 1;
+        if (!LSE)
+            while (gb < 8 && nal * gb * 2 <= NT) gb <<= 1;
+        const int subb = tid & (gb - 1);
+        for (int idx = tid / gb; idx < nal; idx += NT / gb) {
             const int e0 = alist[idx];
             const int i = e0 & 0x7fff;
             const int d = s + (e0 >> 15);
@@ -2030,15 +2056,15 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
             const int pmax = min(i + CF_CMAX, j);
             const double e_stack = bp64[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
             const double sn01 = T[CF_O_B01 + sj], sn10 = T[CF_O_B01 + si1], sn11 = T[CF_O_I11 + cf_i2(si1, sj)];
-            int p = i, qmin = max(i + 2, j - CF_CMAX);
-            uint32_t bits = cf_window(pmask + rs8[p + 1] * W, qmin, j);
-            bool more = true;
+            int p = i + subb, qmin = max(p + 2, p - i + j - CF_CMAX);
+            uint32_t bits = p <= pmax ? cf_window(pmask + rs8[p + 1] * W, qmin, j) : 0u;
+            bool more = p <= pmax;
             while (more) {
                 double fv[8], ev[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    while (bits == 0u && p < pmax) {
-                        ++p;
+                    while (bits == 0u && p + gb <= pmax) {
+                        p += gb;
                         qmin = max(p + 2, p - i + j - CF_CMAX);
                         bits = cf_window(pmask + rs8[p + 1] * W, qmin, j);
                     }
@@ -2067,10 +2093,17 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                     fv[u] = fc_t[off];
                     ev[u] = e;
                 }
-                more = (bits != 0u) || (p < pmax);
+                more = (bits != 0u) || (p + gb <= pmax);
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     if (fv[u] > CF_HALF64) cf_acc<LSE>(best, ev[u] + fv[u]);
+            }
+            if (!LSE) {
+                for (int off = 1; off < gb; off <<= 1) {
+                    const double o = __shfl_xor(best, off);
+                    best = o > best ? o : best;
+                }
+                if (subb != 0) continue;
             }
             const double fm2 = fm2b[(d & 1) * FS + i];
             if (fm2 > CF_HALF64) {
