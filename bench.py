@@ -120,7 +120,12 @@ def run_config(name, args):
         # 20,000 x U[30,500] nt with 2% N: MEA at 3 gammas + Viterbi, length buckets, hipGraph, autotuned
         seqs = workloads.uneven(args.n if args.n != 10000 else 20000, seed=3)
         flags = cf.CF_MEA | cf.CF_MFE
-        tuned = autotune.tune(eng, seqs, flags=flags, gammas=(0.5, 6.0, 16.0), sample=1500, repeats=1, log=log)
+        if args.no_tune:
+            # the built-in tuning (for kernel profiles that should hold only this workload's launches)
+            tuned = {"edges": list(autotune.DEFAULT[0]), "threads": list(autotune.DEFAULT[1]),
+                     "packing": list(autotune.PACKINGS[0])}
+        else:
+            tuned = autotune.tune(eng, seqs, flags=flags, gammas=(0.5, 6.0, 16.0), sample=1500, repeats=1, log=log)
         with eng.batch(seqs) as b:
             b.run(flags | CF_GRAPH, gammas=[0.5, 6.0, 16.0])
             ms = []
@@ -129,9 +134,13 @@ def run_config(name, args):
                 b.run(flags | CF_GRAPH, gammas=[0.5, 6.0, 16.0])
                 ms.append(b.last_ms())
             dt = (time.perf_counter() - t) / args.steps
+            # device ms per stage, one stage at a time (a separate, untimed run)
+            b.run(flags, gammas=[0.5, 6.0, 16.0], recompute=True, timing=True)
+            stages = {k: round(v, 2) for k, v in b.kernel_ms().items() if v > 0.01}
         out.update({"folds_per_s": round(len(seqs) / dt, 1), "device_ms": round(float(np.mean(ms)), 2),
+                    "stages_ms": stages,
                     "tuning": {"edges": tuned["edges"], "threads": tuned["threads"], "packing": tuned["packing"]},
-                    "graph": True})
+                    "tuned": not args.no_tune, "graph": True})
         eng.set_tuning(*autotune.DEFAULT)
         eng.set_packing(*autotune.PACKINGS[0])
     elif name == "c4":
@@ -356,6 +365,7 @@ def main():
     ap.add_argument("--length", type=int, default=200)
     ap.add_argument("--samples", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="c3: keep the built-in tuning instead of autotuning")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the bench line")
     ap.add_argument("--probe", action="store_true",
                     help="for rocprofv3 counter passes: a copy kernel of known size (256 MiB read, 256 MiB "
