@@ -525,6 +525,8 @@ int ensure_lanes(cf_batch* b) {
         b->lane_ev.push_back(e);
     }
     if (!b->fork_ev) CF_HIP(c, hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "fork event");
+    if (!b->vlane) CF_HIP(c, hipStreamCreateWithFlags(&b->vlane, hipStreamNonBlocking), "viterbi lane");
+    if (!b->vlane_ev) CF_HIP(c, hipEventCreateWithFlags(&b->vlane_ev, hipEventDisableTiming), "viterbi event");
     return CF_OK;
 }
 
@@ -538,6 +540,25 @@ int ensure_lanes(cf_batch* b) {
 int enqueue_run(cf_batch* b, unsigned need, uint64_t seed, bool timing, bool fork_ok) {
     cf_ctx* c = b->ctx;
     int rc;
+    // The FP64 Viterbi / exact-logZ sweep reads only the codes and masks: when
+    // the run also has FP32 stages, it runs on its own stream beside them from
+    // the start (split teams excluded: their workgroups must be co-resident).
+    bool split_any = false;
+    for (const auto& bk : b->buckets) split_any |= bk.split > 1;
+    const unsigned fp64 = need & (CF_MFE | CF_LOGZ64);
+    const bool vit_lane = !timing && fork_ok && b->allow_fork && !split_any && fp64 != 0u &&
+                          (need & (CF_LOGZ | CF_BPP | CF_MEA | CF_SAMPLE | CF_GRAD)) != 0u;
+    if (vit_lane) {
+        if ((rc = ensure_lanes(b))) return rc;
+        CF_HIP(c, hipEventRecord(b->fork_ev, b->stream), "fork");
+        CF_HIP(c, hipStreamWaitEvent(b->vlane, b->fork_ev, 0), "fork wait");
+        for (const auto& bk : b->buckets) {
+            if ((need & CF_MFE) && (rc = launch_viterbi(b, bk, false, b->vlane))) return rc;
+            if ((need & CF_LOGZ64) && (rc = launch_viterbi(b, bk, true, b->vlane))) return rc;
+        }
+        CF_HIP(c, hipEventRecord(b->vlane_ev, b->vlane), "join");
+        need &= ~fp64;
+    }
     const std::vector<Bucket> parts =
         (!timing && fork_ok && b->allow_fork) ? fork_parts(b) : std::vector<Bucket>();
     const int np = (int)parts.size();
@@ -605,6 +626,7 @@ int enqueue_run(cf_batch* b, unsigned need, uint64_t seed, bool timing, bool for
     if (timing) (void)hipEventRecord(b->ev[7], b->stream);
     if ((need & CF_GRAD) && (rc = launch_grad(b, b->stream))) return rc;
     if (timing) (void)hipEventRecord(b->ev[8], b->stream);
+    if (vit_lane) CF_HIP(c, hipStreamWaitEvent(b->stream, b->vlane_ev, 0), "join wait");
     return CF_OK;
 }
 
@@ -1389,6 +1411,8 @@ void batch_free(cf_batch* b, bool sync) {
     for (auto e : b->lane_ev)
         if (e) (void)hipEventDestroy(e);
     if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+    if (b->vlane) (void)hipStreamDestroy(b->vlane);
+    if (b->vlane_ev) (void)hipEventDestroy(b->vlane_ev);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (auto& ev : b->ev)

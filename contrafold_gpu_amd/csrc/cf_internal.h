@@ -150,6 +150,8 @@ struct cf_batch {
     std::vector<hipStream_t> lanes;
     std::vector<hipEvent_t> lane_ev;
     hipEvent_t fork_ev = nullptr;
+    hipStream_t vlane = nullptr;  // the FP64 Viterbi sweep beside the FP32 stages
+    hipEvent_t vlane_ev = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     unsigned g_flags = 0;

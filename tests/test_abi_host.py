@@ -290,6 +290,21 @@ def test_autotune_with_a_fake_engine(tmp_path, monkeypatch):
     near = autotune.tune(fm, seqs, sample=1500, repeats=1, use_cache=False)
     FakeBatch.last_ms = fm_batch_ms
     assert near["packing"] == list(autotune.PACKINGS[0]) and near["ms"] == 5.0
+    # a packing whose first timing passes the bar but whose confirmation runs
+    # do not (a lucky sample timing) does not replace the default
+    fl = FakeEngine()
+    seen = {"n": 0}
+
+    def lucky(self):
+        base = 5.0 if self.eng.cur[1][-1] == 256 else 9.0
+        if self.eng.pack == (2, 2):
+            seen["n"] += 1
+            return base - 1.0 if seen["n"] == 1 else base + 0.5
+        return base
+    FakeBatch.last_ms = lucky
+    once = autotune.tune(fl, seqs, sample=1500, repeats=1, use_cache=False)
+    FakeBatch.last_ms = fm_batch_ms
+    assert seen["n"] == 3 and once["packing"] == list(autotune.PACKINGS[0]) and once["ms"] == 5.0
     # no packed bucket among the candidates: no packing phase, default packing
     fw = FakeEngine()
     wide = autotune.tune(fw, seqs, sample=1500, repeats=1, use_cache=False,

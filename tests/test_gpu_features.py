@@ -409,8 +409,9 @@ def test_autotune_installs_fastest(engine, P, tmp_path, monkeypatch):
     base = min(t["ms"] for t in res["tried"] if t["packing"] == list(packs[0]))
     if res["packing"] == list(packs[0]):
         assert res["ms"] == base
-    else:
-        assert res["ms"] < base * (1.0 - autotune.PACKING_MARGIN)
+    else:  # it passed the bar on its first timing, then was confirmed
+        assert any(t["packing"] == res["packing"] and t["ms"] < base * (1.0 - autotune.PACKING_MARGIN)
+                   for t in res["tried"])
     again = autotune.tune(engine, seqs, flags=CF_BPP, sample=60, repeats=1, candidates=cands, packings=packs)
     assert again["cached"] and again["edges"] == res["edges"] and again["packing"] == res["packing"]
     # results do not depend on the tuning or the packing
