@@ -120,8 +120,17 @@ def run_config(name, args):
         # 20,000 x U[30,500] nt with 2% N: MEA at 3 gammas + Viterbi, length buckets, hipGraph, autotuned
         seqs = workloads.uneven(args.n if args.n != 10000 else 20000, seed=3)
         flags = cf.CF_MEA | cf.CF_MFE
-        if args.no_tune:
-            # the built-in tuning (for kernel profiles that should hold only this workload's launches)
+        if args.tuning:
+            # a given tuning, no tuner runs (for kernel profiles that should hold
+            # only this workload's launches): "EDGES:WIDTHS", e.g. "512:64,256"
+            e_s, w_s = args.tuning.split(":")
+            edges = [int(x) for x in e_s.split(",") if x] + [autotune.BIG]
+            widths = [int(x) for x in w_s.split(",")]
+            eng.set_tuning(edges, widths)
+            eng.set_packing(*autotune.PACKINGS[0])
+            tuned = {"edges": edges, "threads": widths, "packing": list(autotune.PACKINGS[0])}
+        elif args.no_tune:
+            # the built-in tuning
             tuned = {"edges": list(autotune.DEFAULT[0]), "threads": list(autotune.DEFAULT[1]),
                      "packing": list(autotune.PACKINGS[0])}
         else:
@@ -140,7 +149,7 @@ def run_config(name, args):
         out.update({"folds_per_s": round(len(seqs) / dt, 1), "device_ms": round(float(np.mean(ms)), 2),
                     "stages_ms": stages,
                     "tuning": {"edges": tuned["edges"], "threads": tuned["threads"], "packing": tuned["packing"]},
-                    "tuned": not args.no_tune, "graph": True})
+                    "tuned": not (args.no_tune or args.tuning), "graph": True})
         eng.set_tuning(*autotune.DEFAULT)
         eng.set_packing(*autotune.PACKINGS[0])
     elif name == "c4":
@@ -366,6 +375,8 @@ def main():
     ap.add_argument("--samples", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="c3: keep the built-in tuning instead of autotuning")
+    ap.add_argument("--tuning", default="", help="c3: use this tuning instead of autotuning, EDGES:WIDTHS "
+                                                 "(e.g. 512:64,256 = folds up to 512 nt packed, longer 256-wide)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the bench line")
     ap.add_argument("--probe", action="store_true",
                     help="for rocprofv3 counter passes: a copy kernel of known size (256 MiB read, 256 MiB "
