@@ -101,16 +101,20 @@ def run_config(name, args):
             for _ in range(3):
                 b.run(cf.CF_MEA | cf.CF_MFE, gammas=[6.0], recompute=True)
             dt = (time.perf_counter() - t) / 3
+            run_ms = b.last_ms()
             z = b.logz()
             mea = b.mea(0)
             vit, _ = b.mfe()
+            # device ms per stage, one stage at a time (a separate, untimed run)
+            b.run(cf.CF_MEA | cf.CF_MFE, gammas=[6.0], recompute=True, timing=True)
+            stages = {k: round(v, 2) for k, v in b.kernel_ms().items() if v > 0.01}
         from oracle import oracle as O
         O.build()
         k = min(args.check, len(seqs))
         z64, _ = O.run_batch("logz64", seqs[:k], P)
         zz, pairs = O.run_batch("mea32", seqs[:k], P, gamma=6.0)
         _v, vp = O.run_batch("viterbi64", seqs[:k], P)
-        out.update({"folds_per_s": round(len(seqs) / dt, 1),
+        out.update({"folds_per_s": round(len(seqs) / dt, 1), "device_ms": round(run_ms, 2), "stages_ms": stages,
                     "logz_max_abs_err_vs_fp32_oracle": float(np.abs(z[:k] - zz.astype(np.float32)).max()),
                     "logz_max_abs_err_vs_fp64": float(np.abs(z[:k].astype(np.float64) - z64).max()),
                     "mea_identical": int(sum(mea[i] == O.dot_bracket(pairs[i], 128) for i in range(k))),
