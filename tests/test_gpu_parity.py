@@ -261,3 +261,68 @@ def test_packed_layout_parity(P, oracle):
                 assert got[k] == oracle.dot_bracket(pair, L), k
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("width", [64, 256], ids=["packed", "wide256"])
+def test_mea_tile_boundaries(P, oracle, width):
+    """The tiled MEA decode (16 x 16 tiles by block diagonal, far terms as tile
+    products, near terms in an in-tile wavefront, traceback by replay) at
+    lengths on both sides of every tile edge up to four tile rows, and a few
+    longer ones, with masks and N bases, three gammas in one run (the items
+    of one sequence share its posteriors): every structure equals the
+    oracle's decode of the oracle's posteriors."""
+    from contrafold_gpu_amd import CF_MEA, Engine
+    rng = np.random.default_rng(1616)
+    lengths = [L for L in range(1, 66)] + [79, 80, 81, 127, 128, 129, 255, 256, 257, 511, 512, 513]
+    seqs, masks = [], []
+    for k, L in enumerate(lengths):
+        s = rand_seq(rng, L)
+        if k % 7 == 3:
+            s = "".join(c if rng.random() > 0.05 else "N" for c in s)
+        seqs.append(s)
+        masks.append((rng.random(L) < 0.1).astype(np.int8) if k % 3 == 1 else None)
+    gammas = [0.5, 6.0, 16.0]
+    eng = Engine()
+    eng.set_params(P)
+    eng.set_tuning([1 << 30], [width])
+    try:
+        with eng.batch(seqs, masks) as b:
+            b.run(CF_MEA, gammas=gammas)
+            got = [b.mea(g) for g in range(len(gammas))]
+            for k, (s, m) in enumerate(zip(seqs, masks)):
+                _, POST = oracle.posterior32(s, P, m)
+                for g_i, g in enumerate(gammas):
+                    pair, _ = oracle.mea_decode32(POST, s, g, m)
+                    assert got[g_i][k] == oracle.dot_bracket(pair, len(s)), (width, len(s), g)
+    finally:
+        eng.close()
+
+
+def test_viterbi_lane_groups_and_stream(P, oracle):
+    """Viterbi with lane groups on short diagonals (groups of 2-8 lanes share a
+    cell's split terms and window rows; FP64 maximum, order-free) and on its
+    own stream beside the FP32 stages: structures and FP64 scores equal the
+    oracle whether Viterbi runs alone, beside an MEA run, or in a replayed
+    hipGraph of that run."""
+    from contrafold_gpu_amd import CF_MEA, CF_MFE, Engine
+    from contrafold_gpu_amd._lib import CF_GRAPH
+    rng = np.random.default_rng(808)
+    seqs, masks = [], []
+    for k, L in enumerate(list(range(1, 41)) + [64, 97, 150, 200, 333]):
+        seqs.append(rand_seq(rng, L))
+        masks.append((rng.random(L) < 0.1).astype(np.int8) if k % 4 == 2 else None)
+    want = [oracle.viterbi64(s, P, m) for s, m in zip(seqs, masks)]
+    eng = Engine()
+    eng.set_params(P)
+    try:
+        with eng.batch(seqs, masks) as b:
+            runs = [(CF_MFE, {}), (CF_MEA | CF_MFE, dict(gammas=[6.0])),
+                    (CF_MEA | CF_MFE | CF_GRAPH, dict(gammas=[6.0])), (CF_MEA | CF_MFE | CF_GRAPH, dict(gammas=[6.0]))]
+            for flags, kw in runs:
+                b.run(flags, recompute=not (flags & CF_GRAPH), **kw)
+                got, scores = b.mfe()
+                for k, (s, (pair, sc)) in enumerate(zip(seqs, want)):
+                    assert got[k] == oracle.dot_bracket(pair, len(s)), (flags, len(s))
+                    assert scores[k] == sc, (flags, len(s))
+    finally:
+        eng.close()
