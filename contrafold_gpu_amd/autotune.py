@@ -38,6 +38,7 @@ DEFAULT = CANDIDATES[0]  # the library's built-in tuning
 # (inside, outside) one-wave teams per workgroup; the first is the default
 PACKINGS = [(8, 4), (4, 4), (8, 8), (16, 4), (4, 2), (2, 2)]
 PACKING_MARGIN = 0.03  # a packing other than the default must be this much faster
+PACKING_SAMPLE = 25000  # the packings are timed on the workload itself, up to this many sequences
 
 
 def cache_path():
@@ -79,7 +80,10 @@ def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, samp
 
     The sample is at least a third of the workload (up to 8,000 sequences):
     a small sample leaves the GPU half empty, where wide teams win on latency,
-    while the whole batch fills it and favours narrower teams."""
+    while the whole batch fills it and favours narrower teams.  The packings
+    are timed on the whole workload (up to PACKING_SAMPLE sequences): on a
+    third of 20,000 x U[30, 500] nt, (2, 2) beat the default (8, 4) by 6% in
+    repeated timings, and lost to it by 7% on the whole batch."""
     from ._lib import CF_BPP, CF_MEA, CF_SAMPLE
     flags = CF_BPP if flags is None else flags
     rng = np.random.default_rng(seed)
@@ -98,8 +102,8 @@ def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, samp
         return {"edges": edges, "threads": threads, "packing": list(packing), "ms": cache[key].get("ms"),
                 "cached": True}
 
-    def timed():
-        with engine.batch(sub, subm) as b:
+    def timed(seqs_=None, masks_=None):
+        with engine.batch(sub if seqs_ is None else seqs_, subm if seqs_ is None else masks_) as b:
             kw = dict(recompute=True)
             if flags & CF_MEA:
                 kw["gammas"] = list(gammas)
@@ -127,18 +131,27 @@ def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, samp
     edges, threads = best[1], best[2]
     engine.set_tuning(edges, threads)
     if 64 in threads:  # a packed bucket: try the other packings on the chosen buckets
-        # Another packing must win clearly: the sample fills the GPU less than
-        # the whole batch, and within a few percent the default holds up better
-        # on the batch (20,000 x U[30, 500] nt: (4, 2) 0.5% faster on the
-        # sample, 5% slower on the batch).
-        # A packing that passes the bar is timed again against the current
-        # choice, alternately, and kept only if it still wins by the margin:
-        # single timings of the sample vary by a few percent (a lucky (2, 2)
-        # once passed on the sample and ran 7% slower on the batch).
+        psize = min(n, max(size, PACKING_SAMPLE))
+        if psize > size:
+            ppick = rng.choice(n, size=psize, replace=False) if psize < n else np.arange(n)
+            psub = [seqs[k] for k in ppick]
+            psubm = [masks[k] for k in ppick] if masks is not None else None
+            engine.set_packing(*packings[0])
+            base_ms = timed(psub, psubm)
+            results.append((base_ms, edges, threads, packings[0]))
+            if log:
+                log(f"[tune] packing={packings[0]} on {psize} sequences: {base_ms:.2f} ms")
+            best = (base_ms, edges, threads, packings[0])
+        else:
+            psub, psubm = None, None
+        # Another packing must win clearly (by PACKING_MARGIN), and one that
+        # passes the bar is timed again against the current choice,
+        # alternately, and kept only if it still wins by the margin: single
+        # timings vary by a few percent.
         bar = best[0] * (1.0 - PACKING_MARGIN)
         for pk in packings[1:]:
             engine.set_packing(*pk)
-            ms = timed()
+            ms = timed(psub, psubm)
             results.append((ms, edges, threads, pk))
             if log:
                 log(f"[tune] packing={pk}: {ms:.2f} ms")
@@ -146,9 +159,9 @@ def tune(engine, seqs, masks=None, flags=None, gammas=(6.0,), n_samples=10, samp
                 cur, new = [], []
                 for _ in range(2):
                     engine.set_packing(*best[3])
-                    cur.append(timed())
+                    cur.append(timed(psub, psubm))
                     engine.set_packing(*pk)
-                    new.append(timed())
+                    new.append(timed(psub, psubm))
                 if log:
                     log(f"[tune] confirm packing={pk}: {min(new):.2f} ms against {best[3]}: {min(cur):.2f} ms")
                 if min(new) < min(cur) * (1.0 - PACKING_MARGIN):

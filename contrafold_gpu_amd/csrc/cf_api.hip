@@ -261,11 +261,14 @@ int launch_post(cf_batch* b, const Bucket& bk, hipStream_t st) {
     if (nl <= 0) return CF_OK;
     // The waves of a workgroup split one sequence's diagonals: the kernel
     // waits on global FCo reads, which more waves per sequence hide better.
-    // Eight per sequence: 27.9 ms at 10,000 x 200 nt, against 30.3 for four
-    // and 41 for two.  A wide bucket's workgroup width (up to 16 waves) splits
-    // long folds further, as far as the per-wave LDS areas fit.
+    // Eight per sequence up to 256 nt (10,000 x 200 nt: 26.0 ms, against 28.1
+    // for four and 41 for two); four above, where a workgroup's per-wave pools
+    // and the longer sequence staging would hold more LDS than they hide
+    // latency (2,000 x 500 nt: 55.8 against 59.4 ms; 20,000 x U[30, 500] nt:
+    // 135 against 155 ms).  A wide bucket's workgroup width (up to 16 waves)
+    // splits long folds further, as far as the per-wave LDS areas fit.
     if (bk.Lmax >= 8192) return fail(c, CF_EINVAL, "sequence too long for the posterior kernel (8191 nt at most)");
-    int split = std::max(8, bk.threads / 64);
+    int split = bk.threads > 64 ? std::max(8, bk.threads / 64) : (bk.Lmax > 256 ? 4 : 8);
     const size_t seq_lds = cf_lds_plan(bk.Lmax, false, 0, 1).total;
     auto lds = [&](int w) { return seq_lds + (size_t)w * CF_POST_WORK; };
     while (lds(split) > (size_t)c->lds_max && split > 1) --split;

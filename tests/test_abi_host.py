@@ -239,7 +239,8 @@ def test_serving_steady_rate():
 def test_autotune_with_a_fake_engine(tmp_path, monkeypatch):
     """Host logic of the tuner: it times every candidate on at least a third of
     the workload (up to 8,000 sequences), then the packings on the fastest one
-    when it has a packed bucket, installs the fastest and caches it."""
+    when it has a packed bucket, on the workload itself (up to 25,000
+    sequences), installs the fastest and caches it."""
     from contrafold_gpu_amd import autotune
     monkeypatch.setenv("CF_TUNE_CACHE", str(tmp_path / "tune.json"))
 
@@ -271,8 +272,8 @@ def test_autotune_with_a_fake_engine(tmp_path, monkeypatch):
     seqs = ["ACGU" * 10] * 9000
     res = autotune.tune(eng, seqs, sample=1500, repeats=1)
     n_cand, n_pack = len(autotune.CANDIDATES), len(autotune.PACKINGS)
-    assert not res["cached"] and len(res["tried"]) == n_cand + n_pack - 1
-    assert set(eng.sizes) == {3000}                      # a third of 9,000, not 1,500
+    assert not res["cached"] and len(res["tried"]) == n_cand + n_pack  # + the default on the whole workload
+    assert set(eng.sizes) == {3000, 9000}  # candidates on a third of 9,000 (not 1,500), packings on all
     assert res["threads"][-1] == 256 and eng.cur == (res["edges"], res["threads"])
     assert 64 in res["threads"] and res["packing"] == [4, 4] and eng.pack == (4, 4) and res["ms"] == 4.0
     assert all(t["packing"] == list(autotune.PACKINGS[0]) for t in res["tried"][:n_cand])
@@ -281,7 +282,7 @@ def test_autotune_with_a_fake_engine(tmp_path, monkeypatch):
     assert again["cached"] and again["threads"] == res["threads"] and eng.pack == (4, 4)
     fe = FakeEngine()
     big = autotune.tune(fe, ["ACGU" * 30] * 60000, sample=1500, repeats=1, use_cache=False)
-    assert big["ms"] == 4.0 and set(fe.sizes) == {8000}  # capped at 8,000
+    assert big["ms"] == 4.0 and set(fe.sizes) == {8000, 25000}  # capped at 8,000 and 25,000
     # a packing within the margin of the default does not replace it
     fm = FakeEngine()
     fm_batch_ms = FakeBatch.last_ms
