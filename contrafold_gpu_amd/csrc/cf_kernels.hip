@@ -498,49 +498,122 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_inside_kernel(const CfSeq
             // Rows p >= i+2: candidates (p ascending, q descending) gathered 8
             // at a time -- their reads do not depend on acc -- then folded in
             // order; exhausted slots hold NEG, which leaves acc unchanged.
-            // Per row, LDS pointers are rebased so a candidate k (q = qlo + k)
-            // indexes them directly: dtab[q - p - 2], qinfo[q], cs[l1*31 + j - q].
-            int p = i + 2, rowJ = 0;
+            // Per row, a candidate k (q = qlo + k) indexes dtab[dlo + k],
+            // qinfo[qlo + k] and cs[clo - k] (dtab[q - p - 2], cs[l1*31 + j - q]).
+            int p = i + 2, rowJ = 0, qlo = 0, dlo = 0, clo = 0;
             uint32_t bits = 0u;
-            const int* dtr = S.dtab;
-            const int* qir = S.qinfo;
-            const float* csr = S.cs;
             auto row = [&]() {
-                const int qlo = max(p + 2, j - CF_CMAX + (p - i));
+                qlo = max(p + 2, j - CF_CMAX + (p - i));
                 rowJ = S.s8[p + 1] * 25 + S.s8[p];
                 bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
-                dtr = S.dtab + (qlo - p - 2);
-                qir = S.qinfo + qlo;
-                csr = S.cs + ((p - i) * 31 + j - qlo);
+                dlo = qlo - p - 2;
+                clo = (p - i) * 31 + j - qlo;
             };
-            if (p <= pend) row();
+            if constexpr (PACK) {
+                if (p <= pend) row();
+            }
             bool more = (p <= pend);
-            // The FC loads of a batch are consumed only in its fold, so all 8 are
-            // in flight together.  An empty slot reads cell (0, 0), which the
-            // first step set to NEG, so its term is NEG.
+            // Wide and split teams (see gather8) hold two rows' partner bits:
+            // the current row (cb, row cp, window start cq, junction index crj)
+            // and the next non-empty one (nb, np, nq, nrj); pr is the next row
+            // to read.
+            uint32_t cb = 0u, nb = 0u;
+            int cp = 0, cq = 0, crj = 0, np = 0, nq = 0, nrj = 0, pr = i + 2;
+            auto load_row = [&](int pp, uint32_t& b, int& q, int& rj) {
+                q = max(pp + 2, j - CF_CMAX + (pp - i));
+                rj = S.s8[pp + 1] * 25 + S.s8[pp];
+                b = cf_window(S.pmask + S.rs8[pp + 1] * S.W, q, j);
+            };
+            // The FC loads of a batch are consumed only in its fold, so all 8
+            // are in flight together.  An empty slot reads FC cell (0, 0),
+            // which the first step set to NEG, so its term is NEG.
             auto gather8 = [&](float* fv, float* ev) {
+                if constexpr (PACK) {
+                    // Packed teams: one slot at a time.  Many resident waves
+                    // hide each slot's LDS latency, and the batched form below
+                    // takes 90 instead of 71 registers (5 instead of 7 waves
+                    // per SIMD), which was slower on batches.
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        while (bits == 0u && p < pend) {
+                            ++p;
+                            row();
+                        }
+                        int off = 0;
+                        float e = 0.0f;
+                        if (bits != 0u) {
+                            const int k = 31 - __clz(bits);
+                            bits &= ~(1u << k);
+                            off = S.dtab[dlo + k] + p + 1;
+                            const float2 bj = S.BJ[(S.qinfo[qlo + k] & 0xffff) + rowJ];
+                            e = S.cs[clo - k] + bj.y;
+                            e = e + jB;
+                            e = e + bj.x;
+                            e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
+                        }
+                        fv[u] = cf_ld(fc_t, off);
+                        ev[u] = e;
+                    }
+                    more = (bits != 0u) || (p < pend);
+                    return;
+                }
+                // Wide and split teams, where one fold's latency is what
+                // counts.  Lanes run out of their rows at different slots, so
+                // a row change inside the slot loop would cost the whole wave
+                // a row's LDS reads at almost every slot.  Instead the two
+                // buffered rows are refilled once per batch (skipping empty
+                // rows), the 8 slots take their candidates from them with
+                // selects only, and then the LDS reads of all 8 slots are
+                // issued together.  A slot is empty (ps < 0, entry 0 of each
+                // table, NEG term) only when both buffered rows run out within
+                // the batch, which the short last rows of a window can do.
+                if (cb == 0u) {
+                    cb = nb; cp = np; cq = nq; crj = nrj;
+                    nb = 0u;
+                }
+                while (cb == 0u && pr <= pend) {
+                    load_row(pr, cb, cq, crj);
+                    cp = pr++;
+                }
+                while (nb == 0u && pr <= pend) {
+                    load_row(pr, nb, nq, nrj);
+                    np = pr++;
+                }
+                int qs[8], ps[8], rj[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    while (bits == 0u && p < pend) {
-                        ++p;
-                        row();
-                    }
-                    int off = 0;
-                    float e = 0.0f;
-                    if (bits != 0u) {
-                        const int k = 31 - __clz(bits);
-                        bits &= ~(1u << k);
-                        off = dtr[k] + p + 1;
-                        const float2 bj = S.BJ[(qir[k] & 0xffff) + rowJ];
-                        e = csr[-k] + bj.y;
-                        e = e + jB;
-                        e = e + bj.x;
-                        e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
-                    }
-                    fv[u] = cf_ld(fc_t, off);
+                    const bool sw = cb == 0u;
+                    cb = sw ? nb : cb;
+                    cp = sw ? np : cp;
+                    cq = sw ? nq : cq;
+                    crj = sw ? nrj : crj;
+                    nb = sw ? 0u : nb;
+                    const int k = 31 - __clz(cb | 1u);  // highest set bit (0 if none)
+                    ps[u] = cb != 0u ? cp : -1;
+                    cb &= ~(1u << k);
+                    qs[u] = cq + k;
+                    rj[u] = crj;
+                }
+                int dv[8], qv[8];
+                float cv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool h = ps[u] >= 0;
+                    dv[u] = S.dtab[h ? qs[u] - ps[u] - 2 : 0];
+                    qv[u] = S.qinfo[h ? qs[u] : 0];
+                    cv[u] = S.cs[h ? (ps[u] - i) * 31 + j - qs[u] : 0];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const float2 bj = S.BJ[(qv[u] & 0xffff) + rj[u]];
+                    float e = cv[u] + bj.y;
+                    e = e + jB;
+                    e = e + bj.x;
+                    e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
+                    fv[u] = cf_ld(fc_t, ps[u] < 0 ? 0 : dv[u] + ps[u] + 1);
                     ev[u] = e;
                 }
-                more = (bits != 0u) || (p < pend);
+                more = (cb != 0u) || (nb != 0u) || (pr <= pend);
             };
             auto fold8 = [&](const float* fv, const float* ev) {
 #pragma unroll
@@ -1130,9 +1203,94 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
                     csr = S.cs + l1 * 31;
                 };
                 if (l1 < l1_last) return;
-                row();
+                if constexpr (PACK) row();
                 bool more = true;
+                // Wide and split teams (see gather8) hold two rows' partner
+                // bits: the current row (cb, row cl, junction index crj) and
+                // the next non-empty one (nb, nl, nrj); lr is the next row to
+                // read (rows run l1 descending).
+                uint32_t cb = 0u, nb = 0u;
+                int cl = 0, crj = 0, nl = 0, nrj = 0, lr = l1;
+                auto load_row = [&](int ll, uint32_t& b, int& rj) {
+                    const int ipn = a - 1 - ll;
+                    b = (ll == 0) ? (bits_r0 & ~1u)
+                                  : cf_window(S.pmask + S.rs8[ipn] * S.W, qbase, min(L, qbase + CF_CMAX - ll));
+                    rj = S.s8[ipn] * 125 + S.s8[ipn + 1] * 5;
+                };
                 auto gather8 = [&](float* fv, float* ev) {
+                    if constexpr (!PACK) {
+                        // Wide and split teams, where one fold's latency is
+                        // what counts (the packed teams keep the one-slot-at-
+                        // a-time form below, whose LDS latency other resident
+                        // waves hide, in fewer registers).  The two buffered
+                        // rows are refilled once per batch (skipping empty
+                        // rows), the 8 slots take their candidates from them
+                        // with selects only -- a row change inside the slot
+                        // loop would cost the wave a row's LDS reads at almost
+                        // every slot -- and then the slots' LDS reads are
+                        // issued together.  A slot is empty (l1s < 0: entry 0
+                        // of each table, NEG term) only when both buffered
+                        // rows run out within the batch.
+                        if (cb == 0u) {
+                            cb = nb; cl = nl; crj = nrj;
+                            nb = 0u;
+                        }
+                        while (cb == 0u && lr >= l1_last) {
+                            load_row(lr, cb, crj);
+                            cl = lr--;
+                        }
+                        while (nb == 0u && lr >= l1_last) {
+                            load_row(lr, nb, nrj);
+                            nl = lr--;
+                        }
+                        int l1s[8], l2s[8], rjs[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const bool sw = cb == 0u;
+                            cb = sw ? nb : cb;
+                            cl = sw ? nl : cl;
+                            crj = sw ? nrj : crj;
+                            nb = sw ? 0u : nb;
+                            const int k = 31 - __clz(cb | 1u);  // highest set bit (0 if none)
+                            l1s[u] = cb != 0u ? cl : -1;
+                            cb &= ~(1u << k);
+                            l2s[u] = k;
+                            rjs[u] = crj;
+                        }
+                        int dv[8], qv[8];
+                        float cv[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const bool h = l1s[u] >= 0;
+                            dv[u] = S.dtab[h ? d + l1s[u] + 2 + l2s[u] : L];
+                            cv[u] = S.cs[h ? l1s[u] * 31 + l2s[u] : 0];
+                            qv[u] = qor[h ? l2s[u] : 0];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const bool h = l1s[u] >= 0;
+                            const int l1u = l1s[u], l2u = l2s[u];
+                            float e = cv[u] + bpab;
+                            e = e + S.JB[rjs[u] + qv[u]];
+                            e = e + jB2;
+                            if constexpr (decltype(extras)::value) {
+                                const float sn = (l1u == 0) ? (l2u == 1 ? sn01 : 0.0f)
+                                               : (l2u == 0 ? sn10 : (l2u == 1 ? sn11 : 0.0f));
+                                e = e + sn;
+                            } else {
+                                e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
+                            }
+                            if constexpr (RING) {
+                                const float fco = h ? ring[((d + l1u + l2u + 2) % CF_RING) * RS + a - 1 - l1u] : CF_NEG;
+                                fv[u] = fco + e;
+                            } else {
+                                fv[u] = cf_ld(fco_t, h ? dv[u] + a - 1 - l1u : off_neg);
+                                ev[u] = e;
+                            }
+                        }
+                        more = (cb != 0u) || (nb != 0u) || (lr >= l1_last);
+                        return;
+                    }
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         while (bits == 0u && l1 > l1_last) {
