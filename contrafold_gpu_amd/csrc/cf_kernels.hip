@@ -498,25 +498,12 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_inside_kernel(const CfSeq
             // Rows p >= i+2: candidates (p ascending, q descending) gathered 8
             // at a time -- their reads do not depend on acc -- then folded in
             // order; exhausted slots hold NEG, which leaves acc unchanged.
-            // Per row, a candidate k (q = qlo + k) indexes dtab[dlo + k],
-            // qinfo[qlo + k] and cs[clo - k] (dtab[q - p - 2], cs[l1*31 + j - q]).
-            int p = i + 2, rowJ = 0, qlo = 0, dlo = 0, clo = 0;
-            uint32_t bits = 0u;
-            auto row = [&]() {
-                qlo = max(p + 2, j - CF_CMAX + (p - i));
-                rowJ = S.s8[p + 1] * 25 + S.s8[p];
-                bits = cf_window(S.pmask + S.rs8[p + 1] * S.W, qlo, j);
-                dlo = qlo - p - 2;
-                clo = (p - i) * 31 + j - qlo;
-            };
-            if constexpr (PACK) {
-                if (p <= pend) row();
-            }
-            bool more = (p <= pend);
-            // Wide and split teams (see gather8) hold two rows' partner bits:
-            // the current row (cb, row cp, window start cq, junction index crj)
-            // and the next non-empty one (nb, np, nq, nrj); pr is the next row
-            // to read.
+            // The walk holds two rows' partner bits: the current row (cb, row
+            // cp, window start cq, junction index crj) and the next non-empty
+            // one (nb, np, nq, nrj); pr is the next row to read.  A candidate
+            // q = cq + k indexes dtab[q - cp - 2], qinfo[q] and
+            // cs[l1*31 + j - q] (l1 = cp - i).
+            bool more = (i + 2 <= pend);
             uint32_t cb = 0u, nb = 0u;
             int cp = 0, cq = 0, crj = 0, np = 0, nq = 0, nrj = 0, pr = i + 2;
             auto load_row = [&](int pp, uint32_t& b, int& q, int& rj) {
@@ -529,24 +516,39 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_inside_kernel(const CfSeq
             // which the first step set to NEG, so its term is NEG.
             auto gather8 = [&](float* fv, float* ev) {
                 if constexpr (PACK) {
-                    // Packed teams: one slot at a time.  Many resident waves
-                    // hide each slot's LDS latency, and the batched form below
-                    // takes 90 instead of 71 registers (5 instead of 7 waves
-                    // per SIMD), which was slower on batches.
+                    // Packed teams: two buffered rows as below, but each
+                    // slot's LDS reads follow its walk step (many resident
+                    // waves hide their latency, and the batched reads take 90
+                    // instead of 71 registers: 5 instead of 7 waves per SIMD)
+                    if (cb == 0u) {
+                        cb = nb; cp = np; cq = nq; crj = nrj;
+                        nb = 0u;
+                    }
+                    while (cb == 0u && pr <= pend) {
+                        load_row(pr, cb, cq, crj);
+                        cp = pr++;
+                    }
+                    while (nb == 0u && pr <= pend) {
+                        load_row(pr, nb, nq, nrj);
+                        np = pr++;
+                    }
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        while (bits == 0u && p < pend) {
-                            ++p;
-                            row();
-                        }
+                        const bool sw = cb == 0u;
+                        cb = sw ? nb : cb;
+                        cp = sw ? np : cp;
+                        cq = sw ? nq : cq;
+                        crj = sw ? nrj : crj;
+                        nb = sw ? 0u : nb;
                         int off = 0;
                         float e = 0.0f;
-                        if (bits != 0u) {
-                            const int k = 31 - __clz(bits);
-                            bits &= ~(1u << k);
-                            off = S.dtab[dlo + k] + p + 1;
-                            const float2 bj = S.BJ[(S.qinfo[qlo + k] & 0xffff) + rowJ];
-                            e = S.cs[clo - k] + bj.y;
+                        if (cb != 0u) {
+                            const int k = 31 - __clz(cb);
+                            cb &= ~(1u << k);
+                            const int q = cq + k;
+                            off = S.dtab[q - cp - 2] + cp + 1;
+                            const float2 bj = S.BJ[(S.qinfo[q] & 0xffff) + crj];
+                            e = S.cs[(cp - i) * 31 + j - q] + bj.y;
                             e = e + jB;
                             e = e + bj.x;
                             e = e + 0.0f;  // the reference adds its (zero) bulge/1x1 extra here too
@@ -554,7 +556,7 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_inside_kernel(const CfSeq
                         fv[u] = cf_ld(fc_t, off);
                         ev[u] = e;
                     }
-                    more = (bits != 0u) || (p < pend);
+                    more = (cb != 0u) || (nb != 0u) || (pr <= pend);
                     return;
                 }
                 // Wide and split teams, where one fold's latency is what
@@ -1189,26 +1191,13 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
             }
             const int16_t* qor = S.qo + (qbase - 1);
             auto walk = [&](auto extras, int l1, int l1_last) {
-                int ip = 0, rowJ = 0;
-                uint32_t bits = 0u;
-                const int* dtr = S.dtab;
-                const float* csr = S.cs;
-                // per row, candidate l2 indexes dtab[d + l1 + 2 + l2] and cs[l1*31 + l2]
-                auto row = [&]() {
-                    ip = a - 1 - l1;
-                    bits = (l1 == 0) ? (bits_r0 & ~1u)
-                                     : cf_window(S.pmask + S.rs8[ip] * S.W, qbase, min(L, qbase + CF_CMAX - l1));
-                    rowJ = S.s8[ip] * 125 + S.s8[ip + 1] * 5;
-                    dtr = S.dtab + (d + l1 + 2);
-                    csr = S.cs + l1 * 31;
-                };
                 if (l1 < l1_last) return;
-                if constexpr (PACK) row();
                 bool more = true;
-                // Wide and split teams (see gather8) hold two rows' partner
-                // bits: the current row (cb, row cl, junction index crj) and
-                // the next non-empty one (nb, nl, nrj); lr is the next row to
-                // read (rows run l1 descending).
+                // The walk holds two rows' partner bits: the current row (cb,
+                // row cl, junction index crj) and the next non-empty one (nb,
+                // nl, nrj); lr is the next row to read (rows run l1
+                // descending).  A candidate l2 of row l1 indexes
+                // dtab[d + l1 + 2 + l2] and cs[l1*31 + l2].
                 uint32_t cb = 0u, nb = 0u;
                 int cl = 0, crj = 0, nl = 0, nrj = 0, lr = l1;
                 auto load_row = [&](int ll, uint32_t& b, int& rj) {
@@ -1291,25 +1280,41 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
                         more = (cb != 0u) || (nb != 0u) || (lr >= l1_last);
                         return;
                     }
+                    // Packed teams: the same two buffered rows, but each
+                    // slot's LDS reads follow its walk step (many resident
+                    // waves hide their latency, in fewer registers)
+                    if (cb == 0u) {
+                        cb = nb; cl = nl; crj = nrj;
+                        nb = 0u;
+                    }
+                    while (cb == 0u && lr >= l1_last) {
+                        load_row(lr, cb, crj);
+                        cl = lr--;
+                    }
+                    while (nb == 0u && lr >= l1_last) {
+                        load_row(lr, nb, nrj);
+                        nl = lr--;
+                    }
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        while (bits == 0u && l1 > l1_last) {
-                            --l1;
-                            row();
-                        }
+                        const bool sw = cb == 0u;
+                        cb = sw ? nb : cb;
+                        cl = sw ? nl : cl;
+                        crj = sw ? nrj : crj;
+                        nb = sw ? 0u : nb;
                         float e = 0.0f;
                         int off = off_neg;
                         int l2 = 0;
-                        const bool has = bits != 0u;
+                        const bool has = cb != 0u;
                         if (has) {
-                            l2 = 31 - __clz(bits);
-                            bits &= ~(1u << l2);
-                            off = dtr[l2] + ip;
-                            e = csr[l2] + bpab;
-                            e = e + S.JB[rowJ + qor[l2]];
+                            l2 = 31 - __clz(cb);
+                            cb &= ~(1u << l2);
+                            off = S.dtab[d + cl + 2 + l2] + a - 1 - cl;
+                            e = S.cs[cl * 31 + l2] + bpab;
+                            e = e + S.JB[crj + qor[l2]];
                             e = e + jB2;
                             if constexpr (decltype(extras)::value) {
-                                const float sn = (l1 == 0) ? (l2 == 1 ? sn01 : 0.0f)
+                                const float sn = (cl == 0) ? (l2 == 1 ? sn01 : 0.0f)
                                                : (l2 == 0 ? sn10 : (l2 == 1 ? sn11 : 0.0f));
                                 e = e + sn;
                             } else {
@@ -1317,14 +1322,14 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_outside_kernel(
                             }
                         }
                         if constexpr (RING) {
-                            const float fco = has ? ring[((d + l1 + l2 + 2) % CF_RING) * RS + ip] : CF_NEG;
+                            const float fco = has ? ring[((d + cl + l2 + 2) % CF_RING) * RS + a - 1 - cl] : CF_NEG;
                             fv[u] = fco + e;
                         } else {
                             fv[u] = cf_ld(fco_t, off);
                             ev[u] = e;
                         }
                     }
-                    more = (bits != 0u) || (l1 > l1_last);
+                    more = (cb != 0u) || (nb != 0u) || (lr >= l1_last);
                 };
                 auto fold8 = [&](const float* fv, const float* ev) {
 #pragma unroll
