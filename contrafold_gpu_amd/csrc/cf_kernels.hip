@@ -1505,29 +1505,53 @@ __global__ void __launch_bounds__(1024) cf_post_kernel(
                 // A slot holds its FCo index and, packed, its JB index (10
                 // bits) and cs index l1*31 + kk (10 bits); the extras are at
                 // cs indices 1 (0x1), 31 (1x0) and 32 (1x1).
+                // The walk holds two rows' partner bits, the current row (cb,
+                // row cl, junction index crj) and the next non-empty one (nb,
+                // nl, nrj), refilled once per batch; the 8 slots take their
+                // sources from them with selects only.  (A row change inside
+                // the slot loop would cost the wave a row's LDS reads at
+                // almost every slot, since lanes run out of their rows at
+                // different slots.)  A slot is empty only when both buffered
+                // rows run out within the batch.
                 auto rows = [&](auto extras, int r0, int r1, uint32_t bits) {
-                    int l1 = r0;
-                    int rowJ = S.s8[a - 1 - l1] * 125 + S.s8[a - l1] * 5;
+                    auto load_row = [&](int ll, uint32_t& b, int& rj) {
+                        b = window(a, qbase, ll);
+                        rj = S.s8[a - 1 - ll] * 125 + S.s8[a - ll] * 5;
+                    };
+                    uint32_t cb = bits, nb = 0u;
+                    int cl = r0, crj = S.s8[a - 1 - r0] * 125 + S.s8[a - r0] * 5, nl = 0, nrj = 0, lr = r0 + 1;
                     bool more = true;
                     while (more) {
+                        if (cb == 0u) {
+                            cb = nb; cl = nl; crj = nrj;
+                            nb = 0u;
+                        }
+                        while (cb == 0u && lr <= r1) {
+                            load_row(lr, cb, crj);
+                            cl = lr++;
+                        }
+                        while (nb == 0u && lr <= r1) {
+                            load_row(lr, nb, nrj);
+                            nl = lr++;
+                        }
                         int off[8], tag[8];
 #pragma unroll
                         for (int u = 0; u < 8; ++u) {
-                            while (bits == 0u && l1 < r1) {
-                                ++l1;
-                                bits = window(a, qbase, l1);
-                                rowJ = S.s8[a - 1 - l1] * 125 + S.s8[a - l1] * 5;
-                            }
+                            const bool sw = cb == 0u;
+                            cb = sw ? nb : cb;
+                            cl = sw ? nl : cl;
+                            crj = sw ? nrj : crj;
+                            nb = sw ? 0u : nb;
                             off[u] = off_neg;
                             tag[u] = 0;
-                            if (bits != 0u) {
-                                const int kk = __ffs(bits) - 1;
-                                bits &= bits - 1u;
-                                off[u] = S.dtab[s + l1 + kk + 2] + a - 1 - l1;
-                                tag[u] = (rowJ + S.qo[qbase + kk - 1]) | ((l1 * 31 + kk) << 10);
+                            if (cb != 0u) {
+                                const int kk = __ffs(cb) - 1;
+                                cb &= cb - 1u;
+                                off[u] = S.dtab[s + cl + kk + 2] + a - 1 - cl;
+                                tag[u] = (crj + S.qo[qbase + kk - 1]) | ((cl * 31 + kk) << 10);
                             }
                         }
-                        more = (bits != 0u) || (l1 < r1);
+                        more = (cb != 0u) || (nb != 0u) || (lr <= r1);
                         float o[8];
 #pragma unroll
                         for (int u = 0; u < 8; ++u) o[u] = cf_ld(fco_t, off[u]);
@@ -2219,24 +2243,49 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
             const int pmax = min(i + CF_CMAX, j);
             const double e_stack = bp64[cf_i2(si1, sj)] + T[CF_O_STACK + cf_i4(si, sj1, si1, sj)];
             const double sn01 = T[CF_O_B01 + sj], sn10 = T[CF_O_B01 + si1], sn11 = T[CF_O_I11 + cf_i2(si1, sj)];
-            int p = i + subb, qmin = max(p + 2, p - i + j - CF_CMAX);
-            uint32_t bits = p <= pmax ? cf_window(pmask + rs8[p + 1] * W, qmin, j) : 0u;
-            bool more = p <= pmax;
+            // The walk holds two rows' partner bits, the current row (cb, row
+            // cp, window start cq) and the next non-empty one (nb, np, nq),
+            // refilled once per batch; the 8 slots take their candidates from
+            // them with selects only (a row change inside the slot loop would
+            // cost the wave a row's LDS reads at almost every slot).  A slot is
+            // empty only when both buffered rows run out within the batch.
+            auto load_row = [&](int pp, uint32_t& b, int& q) {
+                q = max(pp + 2, pp - i + j - CF_CMAX);
+                b = cf_window(pmask + rs8[pp + 1] * W, q, j);
+            };
+            uint32_t cb = 0u, nb = 0u;
+            int cp = 0, cq = 0, np = 0, nq = 0, pr = i + subb;
+            bool more = pr <= pmax;
             while (more) {
+                if (cb == 0u) {
+                    cb = nb; cp = np; cq = nq;
+                    nb = 0u;
+                }
+                while (cb == 0u && pr <= pmax) {
+                    load_row(pr, cb, cq);
+                    cp = pr;
+                    pr += gb;
+                }
+                while (nb == 0u && pr <= pmax) {
+                    load_row(pr, nb, nq);
+                    np = pr;
+                    pr += gb;
+                }
                 double fv[8], ev[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    while (bits == 0u && p + gb <= pmax) {
-                        p += gb;
-                        qmin = max(p + 2, p - i + j - CF_CMAX);
-                        bits = cf_window(pmask + rs8[p + 1] * W, qmin, j);
-                    }
+                    const bool sw = cb == 0u;
+                    cb = sw ? nb : cb;
+                    cp = sw ? np : cp;
+                    cq = sw ? nq : cq;
+                    nb = sw ? 0u : nb;
                     int64_t off = 0;
                     double e = 0.0;
-                    if (bits != 0u) {
-                        const int k = 31 - __clz(bits);
-                        bits &= ~(1u << k);
-                        const int q = qmin + k;
+                    if (cb != 0u) {
+                        const int p = cp;
+                        const int k = 31 - __clz(cb);
+                        cb &= ~(1u << k);
+                        const int q = cq + k;
                         const int l1 = p - i, l2 = j - q;
                         off = cf_tri(L, p + 1, q - 1);
                         if (l1 == 0 && l2 == 0) {
@@ -2256,7 +2305,7 @@ __global__ void __launch_bounds__(256) cf_viterbi_kernel(const CfSeq* __restrict
                     fv[u] = fc_t[off];
                     ev[u] = e;
                 }
-                more = (bits != 0u) || (p + gb <= pmax);
+                more = (cb != 0u) || (nb != 0u) || (pr <= pmax);
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     if (fv[u] > CF_HALF64) cf_acc<LSE>(best, ev[u] + fv[u]);
