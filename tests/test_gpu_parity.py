@@ -32,6 +32,14 @@ def cases(seed=11):
     out += [("", None), ("A", None), ("AU", None), ("GGGGAAAACCCC", None), ("N" * 9, None),
             ("gcgcaaagcgcuuuugcgcaaagcgc", None), ("GCGCGCAAAAGCGCGCAAAAGCGC", np.ones(24, np.int8)),
             ("GGGAAACCCAGGGAAACCCUGGGAAACCC", None), (rand_seq(np.random.default_rng(5), 160), None)]
+    # Windows whose rows are mostly empty: the window walks skip runs of empty
+    # rows when they refill their buffered rows.  A-rich with sparse partners;
+    # half the bases forced unpaired; long unpairable runs between helices.
+    r2 = np.random.default_rng(7)
+    sparse = "".join("A" if r2.random() < 0.85 else "UG"[int(r2.integers(0, 2))] for _ in range(140))
+    out += [(sparse, None),
+            (rand_seq(r2, 120), (r2.random(120) < 0.5).astype(np.int8)),
+            ("GGGCGC" + "A" * 40 + "GCGCCC" + "C" * 30 + "GGAUCC" + "A" * 25 + "GGAUCC", None)]
     return out
 
 
