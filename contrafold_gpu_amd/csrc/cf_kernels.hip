@@ -2570,25 +2570,42 @@ __device__ __forceinline__ uint4 cf_philox(uint4 c, uint2 k) {
     }
     return c;
 }
+// Draws of one (sequence, sample) stream: draw n is word n % 4 (in the order
+// x, y, z, w) of the Philox block with counter n / 4.  The wave evaluates 64
+// blocks at once, lane l holding block base + l, so 256 draws cost one vector
+// Philox evaluation and a draw is read from its lane with readlane.  (On the
+// wave-uniform counter alone, each block's ten rounds ran on the scalar unit,
+// which the traceback's own control already keeps busy.)
 struct CfRng {
     uint2 key;
-    uint32_t sample, seqk, ctr;
-    uint4 buf;
-    int have;
+    uint32_t sample, seqk, base, n;
+    uint4 blk;
+    __device__ void fill() {
+        const uint32_t l = threadIdx.x & (WAVE - 1);
+        blk = cf_philox(make_uint4(sample, base + l, seqk, 0x5eedu), key);
+    }
+    __device__ void start() {
+        base = 0;
+        n = 0;
+        fill();
+    }
     __device__ float next() {
-        if (have == 0) {
-            buf = cf_philox(make_uint4(sample, ctr, seqk, 0x5eedu), key);
-            ++ctr;
-            have = 4;
+        if (n == 4 * WAVE) {
+            base += WAVE;
+            n = 0;
+            fill();
         }
-        // take the words in the order x, y, z, w by shifting the buffer down:
-        // a lane-varying pick among the four would be compiled as a dynamic
-        // index into a private-memory copy (scratch writes per refill)
-        const uint32_t x = buf.x;
-        buf.x = buf.y;
-        buf.y = buf.z;
-        buf.z = buf.w;
-        --have;
+        // the four words of the draw's lane into scalar registers, then a
+        // scalar select (a select among the vector words would be compiled as
+        // a dynamic index into a private-memory copy of the block)
+        const int src = (int)(n >> 2);
+        const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)blk.x, src);
+        const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)blk.y, src);
+        const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)blk.z, src);
+        const uint32_t w3 = (uint32_t)__builtin_amdgcn_readlane((int)blk.w, src);
+        const uint32_t c = n & 3u;
+        const uint32_t x = c == 0u ? w0 : c == 1u ? w1 : c == 2u ? w2 : w3;
+        ++n;
         return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f);  // (0, 1)
     }
 };
@@ -2751,8 +2768,7 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
         rng.key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32) ^ (uint32_t)(key >> 32));
         rng.seqk = (uint32_t)key;
         rng.sample = (uint32_t)smp;
-        rng.ctr = 0;
-        rng.have = 0;
+        rng.start();
         // the stack is wave-uniform: every lane stores the same entry, and a
         // popped entry is read into scalar registers.  Depth bound: a popped
         // state is replaced by states over disjoint, non-empty sub-intervals of
