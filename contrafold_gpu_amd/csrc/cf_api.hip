@@ -699,16 +699,20 @@ int batch_create(cf_ctx* c, int n, const char* seqs, const int64_t* offsets, con
             if (end > pos) {
                 const int Lb = b->L[b->order[pos]];
                 int w = width > 0 ? width : c->threads[k];
-                // Built-in tuning, small packed bucket: four folds per workgroup
-                // would leave most CUs idle, and two-wave teams finish sooner
-                // (1,000 x 128 nt, logZ + MEA + Viterbi: 34.2k -> 40.0k folds/s;
-                // measured gains hold up to ~2,000 x 128 nt and ~500 x 200 nt,
-                // beyond which packing wins); at half that size four-wave teams
-                // are faster still (100 x 200 nt, logZ + 100 samples: 13.2 ->
-                // 12.3 ms).
-                const int64_t work = (int64_t)(end - pos) * Lb * Lb;
-                if (width <= 0 && !c->tuned && w == 64 && work <= (int64_t)83000 * c->n_cu)
-                    w = (2 * work <= (int64_t)83000 * c->n_cu) ? 256 : 128;
+                // Built-in tuning, small packed bucket: one wave per fold would
+                // leave most SIMDs idle, so each fold gets as many waves as keep
+                // about two per SIMD over the bucket, at most four and no more
+                // than one diagonal of the fold can use (logZ + MEA + Viterbi,
+                // ms per run, built-in -> this rule: 1,000 x 128 nt 2 waves, as
+                // before; 500 x 200 nt 33.6 -> 29.5 with 4 instead of 2 waves;
+                // 100 x 200 nt 4 waves and 2,000 x 128 nt packed, as before).
+                if (width <= 0 && !c->tuned && w == 64) {
+                    const int nfold = end - pos;
+                    const int by_gpu = (8 * c->n_cu) / std::max(1, nfold);
+                    const int by_len = (Lb + 1 + 63) / 64;
+                    const int waves = std::min(4, std::min(by_gpu, by_len));
+                    if (waves >= 2) w = 64 * waves;
+                }
                 // wide buckets of long folds may split each fold over several
                 // CUs (one 256-thread workgroup, one wave per SIMD, per part)
                 const int K = w > 64 ? bucket_split(c, Lb, end - pos, split) : 1;
