@@ -50,14 +50,37 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // bases' counts, a work estimate), so the fold queues hand out the lightest
 // folds last and the sweeps drain sooner (10,000 x 200 nt, 10% masked:
 // inside 52.2 -> 49.3 ms, outside 71.0 -> 68.3 ms).
+// The counts are four 16-bit fields of one 64-bit sum (A, C, G, U/T; a
+// sequence has at most 32,000 bases), added from a per-character table with
+// the mask applied as a bit mask: no branch and no store per base (a counter
+// array indexed by base, or a branch on the mask, is several times slower).
+struct BaseIncrements {
+    uint64_t v[256];
+    BaseIncrements() {
+        for (int ch = 0; ch < 256; ++ch) {
+            const uint8_t b = encode_base((char)ch);
+            v[ch] = b < 4 ? 1ull << (16 * b) : 0ull;
+        }
+    }
+};
+const BaseIncrements kBaseInc;
+
 void launch_order(int n, const std::vector<int>& L, const char* seqs, const int64_t* offsets,
                   const uint8_t* masks, std::vector<int>& order) {
     std::vector<int64_t> pairs((size_t)n);
     for (int i = 0; i < n; ++i) {
-        int64_t cnt[5] = {0, 0, 0, 0, 0};
-        for (int64_t q = offsets[i]; q < offsets[i + 1]; ++q)
-            ++cnt[(masks && masks[q]) ? 4 : encode_base(seqs[q])];
-        pairs[i] = cnt[0] * cnt[3] + cnt[1] * cnt[2] + cnt[2] * cnt[3];
+        const uint8_t* s = (const uint8_t*)seqs + offsets[i];
+        const int64_t l = offsets[i + 1] - offsets[i];
+        uint64_t acc = 0;
+        if (masks) {
+            const uint8_t* m = masks + offsets[i];
+            for (int64_t q = 0; q < l; ++q) acc += kBaseInc.v[s[q]] & (0ull - (uint64_t)(m[q] == 0));
+        } else {
+            for (int64_t q = 0; q < l; ++q) acc += kBaseInc.v[s[q]];
+        }
+        const int64_t a = (int64_t)(acc & 0xffff), c = (int64_t)((acc >> 16) & 0xffff);
+        const int64_t g = (int64_t)((acc >> 32) & 0xffff), u = (int64_t)(acc >> 48);
+        pairs[i] = a * u + c * g + g * u;
     }
     order.resize(n);
     for (int i = 0; i < n; ++i) order[i] = i;
