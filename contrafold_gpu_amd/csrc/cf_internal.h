@@ -139,8 +139,9 @@ struct cf_batch {
     double* seqcnt = nullptr;  // per-sequence expected counts, n x 2577
     unsigned done = 0;
     uint64_t params_version = 0;
-    // stage marks 0..8, run start, run end, and the start / end of a sampler
-    // launched on a lane stream beside the outside sweep
+    // stage marks 0..8, run start, run end, an unused slot, and the end of a
+    // sampler launched on a lane stream beside the outside sweep (it starts
+    // at stage mark 2)
     hipEvent_t ev[cfi::kNumKernels + 5] = {};
     bool sample_lane = false;  // the last run sampled on lanes[0]
     float last_ms = 0.0f;
