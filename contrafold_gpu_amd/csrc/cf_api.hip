@@ -150,13 +150,15 @@ void clear_xsync(cf_batch* b, const cf_batch::Bucket& bk, hipStream_t st) {
 
 // Persistent packed launch: workgroups the device holds at once (at most
 // one per G folds), and the fold queue's counter cleared in stream order.
+// `spare` workgroups per CU are left to a kernel running beside it.
 template <typename Kern>
-int persistent_grid(cf_ctx* c, Kern kern, int G, size_t lds, int nfold) {
+int persistent_grid(cf_ctx* c, Kern kern, int G, size_t lds, int nfold, int spare = 0) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * G, lds) != hipSuccess || per_cu <= 0) {
         (void)hipGetLastError();
         per_cu = 1;
     }
+    per_cu = std::max(1, per_cu - spare);
     return (int)std::max<int64_t>(1, std::min<int64_t>((nfold + G - 1) / G, (int64_t)per_cu * c->n_cu));
 }
 int* clear_queue(cf_batch* b, const cf_batch::Bucket& bk, int stage, hipStream_t st) {
@@ -248,7 +250,14 @@ int launch_exterior(cf_batch* b, const Bucket& bk, int mode, hipStream_t st) {
     return CF_OK;
 }
 
-int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st) {
+// beside_sampler: the sampler runs on a lane stream beside this launch.  A
+// packed sweep then leaves about two waves per SIMD to it (eight per CU):
+// the sampler's waves use the sweep's idle issue slots better than the
+// sweep's own last two waves do (10,000 x 200 nt, bench step with the
+// sampler: outside 70.1 -> 76.2 ms, sampler 92.7 -> 81.7 ms, step 152.1 ->
+// 147.9 ms; leaving one wave per SIMD: 151.5 ms; three: 157.9 ms against
+// 146.6 for two on another GPU).
+int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st, bool beside_sampler = false) {
     cf_ctx* c = b->ctx;
     const int nl = bk.end - bk.begin;
     if (nl <= 0) return CF_OK;
@@ -259,7 +268,8 @@ int launch_outside(cf_batch* b, const Bucket& bk, hipStream_t st) {
         while (plan.total > c->lds_max && G > 1) plan = cf_lds_plan(bk.Lmax, false, 0, G >>= 1);
         if (plan.total > c->lds_max) return fail(c, CF_EINVAL, "sequence too long for the outside kernel's LDS plan");
         int* q = clear_queue(b, bk, 1, st);
-        const int grid = persistent_grid(c, cf_outside_kernel<false, true, false>, G, plan.total, nl);
+        const int grid = persistent_grid(c, cf_outside_kernel<false, true, false>, G, plan.total, nl,
+                                         beside_sampler ? (8 + G - 1) / G : 0);
         hipLaunchKernelGGL((cf_outside_kernel<false, true, false>), dim3(grid), dim3(64 * G),
                            plan.total, st, b->d_seqs, b->d_order + bk.begin, nl, b->d_codes, b->d_masks, c->dP32,
                            b->FC, b->FM, b->FM1, b->F5, b->F5o, b->FCo, b->FMo, b->FM1o, b->FM2o, bk.Lmax, 1,
@@ -659,7 +669,7 @@ int enqueue_run(cf_batch* b, unsigned need, uint64_t seed, bool timing, bool for
             mark(2);
         }
         for (const auto& bk : b->buckets)
-            if ((need & CF_BPP) && (rc = launch_outside(b, bk, b->stream))) return rc;
+            if ((need & CF_BPP) && (rc = launch_outside(b, bk, b->stream, sample_early))) return rc;
         mark(3);
         for (const auto& bk : b->buckets)
             if ((need & CF_BPP) && (rc = launch_post(b, bk, b->stream))) return rc;
