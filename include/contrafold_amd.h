@@ -128,8 +128,9 @@ int cf_batch_table(const cf_batch *b, int index, int which, float *out);
 float cf_batch_last_ms(const cf_batch *b);
 /* Per-kernel device milliseconds of the last CF_TIMING run; names in
  * cf_kernel_name(k), k < cf_kernel_count().  In a run with logZ, bpp and
- * samples the sampler runs on a lane stream beside the outside sweep and the
- * posterior, so those three stages overlap. */
+ * samples the sampler runs on a lane stream beside the outside sweep (and the
+ * start of the posterior), so those stages overlap; its time runs from the
+ * outside sweep's start to its own end. */
 int cf_kernel_count(void);
 const char *cf_kernel_name(int k);
 int cf_batch_kernel_ms(const cf_batch *b, float *out);
