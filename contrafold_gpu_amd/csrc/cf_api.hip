@@ -401,19 +401,24 @@ int launch_sample(cf_batch* b, uint64_t seed, hipStream_t st) {
     const bool local = samp_local(b->Lmax, masks);
     const size_t lds = (size_t)CF_SAMP_WAVES * cf_samp_pitch(b->Lmax, local, masks);
     // the task counter: the first bucket's fourth queue slot (the sweeps use
-    // slots 0-2), cleared in stream order
+    // slots 0-2), cleared in stream order.  Waves take four tasks at a time
+    // when there are many per wave, one when there are few (100 x 200 nt x
+    // 100 samples: chunks of four left most waves idle and the rest four
+    // samples deep, 0.3 -> 0.7 ms).
     int* q = b->d_queue + 3;
     hipLaunchKernelGGL(cf_fill_u32, dim3(1), dim3(64), 0, st, (unsigned*)q, (int64_t)1, 0u);
+    const int64_t tasks = (int64_t)b->n * b->n_samples;
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(4, tasks / ((int64_t)b->sgrid * CF_SAMP_WAVES * 8)));
     if (local)
         hipLaunchKernelGGL(cf_sample_kernel<true>, dim3(b->sgrid), dim3(CF_SAMP_WAVES * WAVE), lds, st, b->d_seqs,
                            b->d_order, b->n, b->n_samples, seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC,
                            b->FM, b->FM1, b->FM2, b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff, b->Lmax,
-                           b->d_status, q);
+                           b->d_status, q, chunk);
     else
         hipLaunchKernelGGL(cf_sample_kernel<false>, dim3(b->sgrid), dim3(CF_SAMP_WAVES * WAVE), lds, st, b->d_seqs,
                            b->d_order, b->n, b->n_samples, seed, b->d_keys, b->d_codes, b->d_masks, c->dP32, b->FC,
                            b->FM, b->FM1, b->FM2, b->F5, b->stack, b->stack_cap, b->samp_out, b->d_soff, b->Lmax,
-                           b->d_status, q);
+                           b->d_status, q, chunk);
     CF_HIP(c, hipGetLastError(), "sample launch");
     return CF_OK;
 }

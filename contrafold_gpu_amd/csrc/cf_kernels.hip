@@ -2680,7 +2680,7 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
     const CfParams<float>* __restrict__ P, const float* __restrict__ FC, const float* __restrict__ FM,
     const float* __restrict__ FM1, const float* __restrict__ FM2, const float* __restrict__ F5,
     uint32_t* __restrict__ stack, int stack_cap, uint8_t* __restrict__ out, const int64_t* __restrict__ out_off,
-    int lmax, int* __restrict__ status, int* __restrict__ queue) {
+    int lmax, int* __restrict__ status, int* __restrict__ queue, int chunk) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ CfParams<float> sp;
     __shared__ uint8_t wl1[496], wl2[496];  // single-branch window order: l1 ascending, l2 ascending
@@ -2713,21 +2713,20 @@ __global__ void __launch_bounds__(CF_SAMP_WAVES * WAVE, 8) cf_sample_kernel(
     const float* __restrict__ JBt = sp.JB;
     const float mu = T[CF_O_MU], mp = T[CF_O_MP], eu = T[CF_O_EU], ep = T[CF_O_EP];
     int prev_b = -1;
-    // Tasks are (sequence, sample) pairs in sequence-major order, taken four
-    // at a time from a counter: waves that are resident keep taking work while
-    // other workgroups wait for room beside the outside sweep, and a wave
-    // mostly stays on one sequence, refilling its LDS rows once per chunk
-    // (10,000 x 200 nt x 100 beside the outside sweep: 80.5 -> 71.0 ms, bench
-    // step 146.5 -> 144.9 ms; samples unchanged: each task's stream depends
-    // only on the sequence key and the sample index).
-    constexpr int kChunk = 4;
+    // Tasks are (sequence, sample) pairs in sequence-major order, taken
+    // `chunk` (1-4) at a time from a counter: waves that are resident keep
+    // taking work while other workgroups wait for room beside the outside
+    // sweep, and a wave mostly stays on one sequence, refilling its LDS rows
+    // once per chunk (10,000 x 200 nt x 100 beside the outside sweep: 80.5 ->
+    // 71.0 ms, bench step 146.5 -> 144.9 ms; samples unchanged: each task's
+    // stream depends only on the sequence key and the sample index).
     auto next_chunk = [&]() -> int64_t {
         int v = 0;
-        if (lane == 0) v = atomicAdd(queue, kChunk);
+        if (lane == 0) v = atomicAdd(queue, chunk);
         return (int64_t)__builtin_amdgcn_readfirstlane(v);
     };
     for (int64_t c0 = next_chunk(); c0 < total; c0 = next_chunk())
-    for (int64_t task = c0, tend = c0 + kChunk < total ? c0 + kChunk : total; task < tend; ++task) {
+    for (int64_t task = c0, tend = c0 + chunk < total ? c0 + chunk : total; task < tend; ++task) {
         const int li = (int)(task / nsamp);
         const int smp = (int)(task % nsamp);
         const int b = sidx[li];
